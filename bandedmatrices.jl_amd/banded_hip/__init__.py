@@ -1,0 +1,18 @@
+"""banded_hip: the MI355X (gfx950) banded hot path of BandedMatrices.jl.
+
+The arithmetic runs in libbandedhip.so (HIP kernels behind the C ABI in
+include/bandedhip.h); this package is the host-side mirror of the reference's
+operator interface for the path: BandedMatrix storage, mul!/A*x, A*B, lu,
+ldiv!/\\ and qr.
+"""
+from ._lib import BandedHIPError, load as load_library, LIB_PATH
+from .device import DeviceArray, Timer, sync
+from .bandedmatrix import (BandedMatrix, BandError, DimensionMismatch, _BandedMatrix, brand, gbmm, mul,
+                           prodbandwidths)
+from .lu import BandedLU, SingularException, TransposeFact, ldiv_, lu, lu_, pad_for_lu, solve
+
+__all__ = [
+    "BandedHIPError", "load_library", "LIB_PATH", "DeviceArray", "Timer", "sync", "BandedMatrix", "BandError",
+    "DimensionMismatch", "_BandedMatrix", "brand", "gbmm", "mul", "prodbandwidths", "BandedLU",
+    "SingularException", "TransposeFact", "ldiv_", "lu", "lu_", "pad_for_lu", "solve",
+]

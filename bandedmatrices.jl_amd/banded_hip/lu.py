@@ -1,0 +1,152 @@
+"""Banded LU and solve, mirroring the reference's BandedLU on the hot path.
+
+  BandedLU{T,S}(factors, ipiv, info)  src/banded/BandedLU.jl:10-27
+  lu / lu!                            src/banded/BandedLU.jl:90-111
+  ldiv!(::BandedLU, B) ('N')          src/banded/linalg.jl:24-30
+  ldiv!(transpose(F), B) ('T')        src/banded/linalg.jl:41-47,55-56
+  A \\ b (Ldiv materialize!)          src/banded/linalg.jl:5-9,75
+Factors stay in HBM; ipiv is kept both on the device (for solves) and as the
+host int64 vector the reference's BandedLU carries (BandedLU.jl:12).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .bandedmatrix import BandedMatrix, DimensionMismatch, _prefix
+from .device import DeviceArray
+
+
+class SingularException(Exception):
+    """Zero pivot reported by gbtrf (LAPACK info > 0)."""
+
+    def __init__(self, info):
+        super().__init__(f"SingularException({info})")
+        self.info = info
+
+
+class BandedLU:
+    """factors: the padded BandedMatrix with bands (l, l+u); ipiv 1-based."""
+
+    def __init__(self, factors: BandedMatrix, ipiv_dev: DeviceArray | None, ipiv: np.ndarray, info: int):
+        self.factors = factors
+        self.ipiv_dev = ipiv_dev
+        self.ipiv = ipiv
+        self.info = int(info)
+
+    @property
+    def shape(self):
+        return self.factors.shape
+
+    def size(self, d=None):
+        return self.shape if d is None else self.shape[d - 1]
+
+    def issuccess(self) -> bool:
+        return self.info == 0
+
+    @property
+    def kl(self):
+        return self.factors.l
+
+    @property
+    def ku(self):
+        # the padded factors carry u+l super-diagonals; LAPACK's KU is u
+        return self.factors.u - self.factors.l
+
+    def transpose(self) -> "TransposeFact":
+        return TransposeFact(self)
+
+    @property
+    def T(self):
+        return self.transpose()
+
+    @property
+    def p(self) -> np.ndarray:
+        """Row permutation (0-based) from ipiv, like ipiv2perm."""
+        m = self.shape[0]
+        perm = np.arange(m)
+        for j, q in enumerate(self.ipiv):
+            q -= 1
+            if q != j:
+                perm[j], perm[q] = perm[q], perm[j]
+        return perm
+
+
+class TransposeFact:
+    def __init__(self, parent: BandedLU):
+        self.parent = parent
+
+
+def lu_(A: BandedMatrix, check: bool = True) -> BandedLU:
+    """lu!(A): A must already carry the l extra super-diagonals (bands (l, l+u))."""
+    m, n = A.shape
+    l, uu = A.bandwidths()
+    ku = uu - l
+    p = _prefix(A.dtype)
+    mn = min(m, n)
+    if m == 0:
+        return BandedLU(A, None, np.zeros(0, np.int64), 0)
+    ipiv_dev = DeviceArray((max(mn, 1),), np.int64)
+    info = C.c_int64(0)
+    _lib.call(f"bh_{p}gbtrf", m, n, l, ku, A.data.ptr, A.data.ld, ipiv_dev.ptr, C.byref(info))
+    ipiv = ipiv_dev.numpy()[:mn].astype(np.int64)
+    F = BandedLU(A, ipiv_dev, ipiv, info.value)
+    if check and info.value > 0:
+        raise SingularException(info.value)
+    return F
+
+
+def pad_for_lu(A: BandedMatrix, dtype=None) -> BandedMatrix:
+    """BandedMatrix{float(T)}(A, (l, l+u)) (src/banded/BandedMatrix.jl:222-232)."""
+    m, n = A.shape
+    l, u = A.bandwidths()
+    dt = np.dtype(dtype or A.dtype)
+    ab = DeviceArray((2 * l + u + 1, n), dt)
+    if dt == A.dtype:
+        _lib.call(f"bh_{_prefix(dt)}gbpad", m, n, l, u, A.data.ptr, A.data.ld, ab.ptr, ab.ld)
+    elif A.dtype == np.float64 and dt == np.float32:
+        _lib.call("bh_dsgbpad", m, n, l, u, A.data.ptr, A.data.ld, ab.ptr, ab.ld)
+    else:
+        raise TypeError(f"no padded copy from {A.dtype} to {dt}")
+    return BandedMatrix(ab, m, l, l + u)
+
+
+def lu(A: BandedMatrix, check: bool = True) -> BandedLU:
+    """_lu: pad to (l, l+u), then lu! (src/banded/BandedLU.jl:108-111)."""
+    if A.l < 0 or A.u < 0:
+        raise NotImplementedError("lu of a matrix with negative bandwidths")
+    return lu_(pad_for_lu(A), check=check)
+
+
+def ldiv_(F, B: DeviceArray) -> DeviceArray:
+    """ldiv!(F, B) in place; F a BandedLU or transpose(F)."""
+    trans = "N"
+    if isinstance(F, TransposeFact):
+        F, trans = F.parent, "T"
+    m, n = F.shape
+    if m != n:
+        raise DimensionMismatch(f"matrix is not square: dimensions are {F.shape}")
+    if B.shape[0] != n:
+        raise DimensionMismatch(f"B has leading dimension {B.shape[0]}, but needs {n}")
+    if B.dtype != F.factors.dtype:
+        raise TypeError("mixed element types")
+    if m == 0:
+        return B
+    nrhs = 1 if len(B.shape) == 1 else B.shape[1]
+    p = _prefix(B.dtype)
+    _lib.call(f"bh_{p}gbtrs", trans.encode(), n, F.kl, F.ku, nrhs, F.factors.data.ptr, F.factors.data.ld,
+              F.ipiv_dev.ptr, B.ptr, max(1, B.shape[0]))
+    return B
+
+
+def solve(A: BandedMatrix, b: DeviceArray) -> DeviceArray:
+    """A \\ b: checksquare, factorize, solve a copy of b (src/banded/linalg.jl:5-9)."""
+    m, n = A.shape
+    if m != n:
+        raise DimensionMismatch(f"matrix is not square: dimensions are {A.shape}")
+    if b.shape[0] != m:
+        raise DimensionMismatch(f"b has length {b.shape[0]}, A has {m} rows")
+    x = b.copy()
+    return ldiv_(lu(A), x)

@@ -1,0 +1,166 @@
+// bh_gbtrf.hip -- partial-pivot band LU on gfx950.
+//
+// Restates LAPACK xGBTF2, the routine the reference's LAPACK.gbtrf!
+// (src/banded/BandedLU.jl:98) runs for ku <= 64, step for step:
+//   for each column j: zero the fill-in rows of column j+kv; pick the pivot
+//   as the first row of max |a| among rows j..j+km (IDAMAX); record ipiv;
+//   swap rows across columns j..ju; scale the multipliers by the rounded
+//   reciprocal of the pivot; rank-1 update a_ic = fma(l_i, -u_jc, a_ic).
+// The rounding of every step matches the CPU routine, so factors and pivots
+// are bit-identical to it.
+//
+// Kernel: one workgroup runs the column chain of one system; the working
+// window lives in global memory (L2-resident: at most (2kl+ku+1) x (kl+ku+1)
+// entries are live) and the phases of a step are separated by workgroup
+// barriers.  Each phase is spread over the workgroup's lanes.
+#include "bh_common.hpp"
+
+namespace bh {
+
+constexpr int kLuThreads = 256;
+
+template <typename T>
+struct LuSys {
+    T *ab;
+    i64 *ipiv;
+    i64 m, n, kl, ku, ldab;
+};
+
+// Argmax over |col[kv+t]|, t in [0, km], first index on ties; whole block.
+template <typename T>
+__device__ __forceinline__ i64 block_pivot(const T *col, i64 km, T *sval, int *sidx) {
+    T best = T(-1);
+    int bidx = 0x7fffffff;
+    for (i64 t = threadIdx.x; t <= km; t += blockDim.x) {
+        T v = dabs(col[t]);
+        if (v > best) {  // strictly greater keeps the first index per thread
+            best = v;
+            bidx = (int)t;
+        }
+    }
+    // wave reduction: larger value wins, equal values keep the smaller index
+    for (int off = 32; off > 0; off >>= 1) {
+        T ov = __shfl_xor(best, off, 64);
+        int oi = __shfl_xor(bidx, off, 64);
+        if (ov > best || (ov == best && oi < bidx)) {
+            best = ov;
+            bidx = oi;
+        }
+    }
+    const int wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sval[wid] = best;
+        sidx[wid] = bidx;
+    }
+    __syncthreads();
+    T b = sval[0];
+    int bi = sidx[0];
+    for (int w = 1; w < nw; ++w) {
+        if (sval[w] > b || (sval[w] == b && sidx[w] < bi)) {
+            b = sval[w];
+            bi = sidx[w];
+        }
+    }
+    __syncthreads();
+    return bi == 0x7fffffff ? 0 : bi;
+}
+
+template <typename T>
+__device__ void gbtf2_block(const LuSys<T> s, i64 *info_out) {
+    __shared__ T sval[16];
+    __shared__ int sidx[16];
+    const i64 kl = s.kl, ku = s.ku, kv = kl + ku, ld = s.ldab, m = s.m, n = s.n;
+    T *ab = s.ab;
+    i64 info = 0;
+    if (m == 0 || n == 0) {
+        if (threadIdx.x == 0 && info_out) *info_out = 0;
+        return;
+    }
+    // fill-in rows of columns ku+1 .. min(kv,n)-1 start at zero
+    for (i64 c = ku + 1; c < imin(kv, n); ++c)
+        for (i64 r = kv - c + threadIdx.x; r < kl; r += blockDim.x) ab[c * ld + r] = T(0);
+    __syncthreads();
+    i64 ju = 0;
+    const i64 steps = imin(m, n);
+    for (i64 j = 0; j < steps; ++j) {
+        if (j + kv < n)
+            for (i64 r = threadIdx.x; r < kl; r += blockDim.x) ab[(j + kv) * ld + r] = T(0);
+        const i64 km = imin(kl, m - 1 - j);
+        T *colj = ab + j * ld + kv;  // colj[t] = A(j+t, j)
+        const i64 jp = block_pivot(colj, km, sval, sidx);
+        if (threadIdx.x == 0) s.ipiv[j] = jp + j + 1;
+        const T piv = colj[jp];
+        if (piv != T(0)) {
+            ju = imax(ju, imin(j + ku + jp, n - 1));
+            if (jp != 0) {
+                for (i64 c = j + threadIdx.x; c <= ju; c += blockDim.x) {
+                    T *p = ab + c * ld + kv + jp + j - c;
+                    T *q = ab + c * ld + kv + j - c;
+                    T tmp = *p;
+                    *p = *q;
+                    *q = tmp;
+                }
+                __syncthreads();
+            }
+            if (km > 0) {
+                const T r = T(1) / colj[0];
+                for (i64 t = 1 + threadIdx.x; t <= km; t += blockDim.x) colj[t] = colj[t] * r;
+                __syncthreads();
+                const i64 ncol = ju - j;
+                const i64 tot = km * ncol;
+                for (i64 e = threadIdx.x; e < tot; e += blockDim.x) {
+                    const i64 cc = e / km, t = e - cc * km + 1, c = j + 1 + cc;
+                    T *colc = ab + c * ld;
+                    const T negu = -colc[kv + j - c];
+                    colc[kv + j + t - c] = dfma(colj[t], negu, colc[kv + j + t - c]);
+                }
+            }
+        } else if (info == 0) {
+            info = j + 1;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && info_out) *info_out = info;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kLuThreads) gbtf2_single(LuSys<T> s, i64 *info_out) {
+    gbtf2_block<T>(s, info_out);
+}
+
+template <typename T>
+static int gbtrf_impl(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, i64 *info_host) {
+    BH_ARG(m >= 0, 1);
+    BH_ARG(n >= 0, 2);
+    BH_ARG(kl >= 0, 3);
+    BH_ARG(ku >= 0, 4);
+    BH_ARG(ldab >= 2 * kl + ku + 1, 6);
+    BH_ARG(info_host != nullptr, 8);
+    *info_host = 0;
+    if (m == 0 || n == 0) return BH_OK;
+    i64 *dinfo = nullptr;
+    BH_HIP(hipMallocAsync((void **)&dinfo, sizeof(i64), stream()));
+    LuSys<T> s{ab, ipiv, m, n, kl, ku, ldab};
+    hipLaunchKernelGGL(gbtf2_single<T>, dim3(1), dim3(kLuThreads), 0, stream(), s, dinfo);
+    BH_LAUNCH_CHECK();
+    BH_HIP(hipMemcpyAsync(info_host, dinfo, sizeof(i64), hipMemcpyDeviceToHost, stream()));
+    BH_HIP(hipFreeAsync(dinfo, stream()));
+    BH_HIP(hipStreamSynchronize(stream()));
+    return BH_OK;
+}
+
+}  // namespace bh
+
+using namespace bh;
+
+extern "C" {
+
+int bh_dgbtrf(bh_int m, bh_int n, bh_int kl, bh_int ku, double *AB, bh_int ldab, bh_int *ipiv, bh_int *info_host) {
+    return gbtrf_impl<double>(m, n, kl, ku, AB, ldab, ipiv, info_host);
+}
+
+int bh_sgbtrf(bh_int m, bh_int n, bh_int kl, bh_int ku, float *AB, bh_int ldab, bh_int *ipiv, bh_int *info_host) {
+    return gbtrf_impl<float>(m, n, kl, ku, AB, ldab, ipiv, info_host);
+}
+
+}  // extern "C"

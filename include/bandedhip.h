@@ -1,0 +1,140 @@
+/*
+ * bandedhip.h -- C ABI of libbandedhip.so, the MI355X (gfx950) banded hot path.
+ *
+ * Each entry point replaces one native call (or one Julia loop of native
+ * calls) that BandedMatrices.jl makes on its hot path; the citation next to
+ * each names the reference interface it stands in for.  Conventions follow
+ * the Fortran BLAS/LAPACK interfaces that the reference binds with ccall
+ * (src/blas.jl:16-28, stdlib LAPACK.gbtrf!/gbtrs!):
+ *   - character flags 'N' / 'T' (real types: 'C' is treated as 'T'),
+ *   - 64-bit integer sizes and leading dimensions (BlasInt = Int64),
+ *   - column-major band storage: A[i,j] (0-based) at A[(ku+i-j) + j*lda],
+ *   - every pointer named d* / A / B / C / x / y / ipiv is DEVICE memory
+ *     owned by the caller; host pointers are named *_host,
+ *   - ipiv is 1-based (LAPACK convention) and int64.
+ * Return value: 0 on success; -i when argument i is invalid (LAPACK xerbla
+ * numbering, 1-based); BH_ERR_* (<= -1000) for runtime failures.  Singular
+ * factorisations are not errors: they report info > 0 like LAPACK.
+ * Work is stream-ordered on the library stream (bh_set_stream); calls that
+ * hand a value back to the host (info, iteration counts) synchronise it.
+ */
+#ifndef BANDEDHIP_H
+#define BANDEDHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int64_t bh_int;
+
+#define BH_OK 0
+#define BH_ERR_HIP (-1000)        /* a HIP runtime call failed (see bh_last_error) */
+#define BH_ERR_ALLOC (-1001)      /* device allocation failed */
+#define BH_ERR_NODEVICE (-1002)   /* no usable gfx950 device */
+#define BH_ERR_BANDERROR (-1003)  /* C lacks bands for A*B (BandError, Band.jl:75-88) */
+#define BH_ERR_WORKSPACE (-1004)  /* workspace too small */
+
+/* ---------------------------------------------------------------- runtime */
+/* No reference counterpart: device/memory plumbing the Julia extension uses to
+ * place BandedMatrix data in HBM (the reference's examples do the same with a
+ * GPU array container, examples/clarrays.jl:15). */
+int bh_init(int device);
+int bh_device_count(int *count);
+const char *bh_last_error(void);
+int bh_malloc(void **dptr, size_t bytes);
+int bh_free(void *dptr);
+int bh_memcpy_htod(void *dst, const void *src_host, size_t bytes);
+int bh_memcpy_dtoh(void *dst_host, const void *src, size_t bytes);
+int bh_memcpy_dtod(void *dst, const void *src, size_t bytes);
+int bh_memset(void *dptr, int value, size_t bytes);
+int bh_sync(void);
+int bh_set_stream(void *hip_stream); /* NULL selects the library's own stream */
+void *bh_get_stream(void);
+int bh_event_create(void **ev);
+int bh_event_record(void *ev);        /* on the library stream */
+int bh_event_elapsed_ms(void *start, void *stop, float *ms); /* synchronises stop */
+int bh_event_destroy(void *ev);
+
+/* ------------------------------------------------------------------ gbmv */
+/* y <- alpha*op(A)*x + beta*y, A m x n with bands (kl,ku).  beta == 0
+ * overwrites y (NaN-safe, src/generic/utils.jl:29-31).  'N' reproduces the
+ * rounding of the column-ordered BLAS kernel bit for bit.
+ * Replaces BLAS.gbmv! via banded_gbmv! (src/generic/matmul.jl:21-23) and the
+ * pointer wrapper gbmv! (src/blas.jl:16-28). */
+int bh_dgbmv(char trans, bh_int m, bh_int n, bh_int kl, bh_int ku, double alpha,
+             const double *A, bh_int lda, const double *x, bh_int incx, double beta,
+             double *y, bh_int incy);
+int bh_sgbmv(char trans, bh_int m, bh_int n, bh_int kl, bh_int ku, float alpha,
+             const float *A, bh_int lda, const float *x, bh_int incx, float beta,
+             float *y, bh_int incy);
+
+/* Y <- alpha*op(A)*X + beta*Y for a dense column-major X with nrhs columns,
+ * A read from HBM once.  Each column equals a bh_?gbmv call bit for bit.
+ * Replaces the per-column mul! loop of
+ * materialize!(::MatMulMatAdd{<:BandedColumns,...}) (src/generic/matmul.jl:243-256). */
+int bh_dgbmv_multi(char trans, bh_int m, bh_int n, bh_int kl, bh_int ku, bh_int nrhs,
+                   double alpha, const double *A, bh_int lda, const double *X, bh_int ldx,
+                   double beta, double *Y, bh_int ldy);
+int bh_sgbmv_multi(char trans, bh_int m, bh_int n, bh_int kl, bh_int ku, bh_int nrhs,
+                   float alpha, const float *A, bh_int lda, const float *X, bh_int ldx,
+                   float beta, float *Y, bh_int ldy);
+
+/* -------------------------------------------------------------- gbmm */
+/* C <- alpha*A*B + beta*C, all three banded (A n x k (Al,Au), B k x m (Bl,Bu),
+ * C n x m (Cl,Cu)), non-negative bandwidths.  Each entry follows the rounding
+ * of gbmm!'s per-column gbmv bit for bit.  When C is narrower than the
+ * product, all-zero outer bands of A and B are stripped first, else
+ * BH_ERR_BANDERROR.  Replaces gbmm! (src/banded/gbmm.jl:207-340). */
+int bh_dgbmm(bh_int n, bh_int k, bh_int m, double alpha,
+             const double *A, bh_int lda, bh_int Al, bh_int Au,
+             const double *B, bh_int ldb, bh_int Bl, bh_int Bu,
+             double beta, double *C, bh_int ldc, bh_int Cl, bh_int Cu);
+int bh_sgbmm(bh_int n, bh_int k, bh_int m, float alpha,
+             const float *A, bh_int lda, bh_int Al, bh_int Au,
+             const float *B, bh_int ldb, bh_int Bl, bh_int Bu,
+             float beta, float *C, bh_int ldc, bh_int Cl, bh_int Cu);
+
+/* ------------------------------------------------------------- padding */
+/* AB <- the padded copy of A (m x n, bands (kl,ku)) with kl extra zero
+ * super-diagonals on top: LAPACK's AB layout, ldab >= 2kl+ku+1.  Entries
+ * outside A's band are written as zero.  bh_dsgbpad converts FP64 -> FP32.
+ * Replaces BandedMatrix{T}(A, (l, l+u)) (src/banded/BandedMatrix.jl:222-232)
+ * as used by _lu (src/banded/BandedLU.jl:108-111) and _banded_qr
+ * (src/banded/bandedqr.jl:2). */
+int bh_dgbpad(bh_int m, bh_int n, bh_int kl, bh_int ku, const double *A, bh_int lda, double *AB,
+              bh_int ldab);
+int bh_sgbpad(bh_int m, bh_int n, bh_int kl, bh_int ku, const float *A, bh_int lda, float *AB,
+              bh_int ldab);
+int bh_dsgbpad(bh_int m, bh_int n, bh_int kl, bh_int ku, const double *A, bh_int lda, float *AB,
+               bh_int ldab);
+
+/* ------------------------------------------------------------- gbtrf */
+/* Partial-pivot band LU in place on the padded array AB (ldab >= 2kl+ku+1,
+ * diagonal at row kl+ku), pivots into ipiv (device, 1-based, min(m,n)
+ * entries), LAPACK info into *info_host (0 ok, j>0 zero pivot in column j).
+ * For ku <= 64 the factors and pivots equal LAPACK dgbtf2/dgbtrf bit for bit;
+ * wider bands use a blocked MFMA factorisation whose pivots match.
+ * Replaces LAPACK.gbtrf! in lu! (src/banded/BandedLU.jl:90-103). */
+int bh_dgbtrf(bh_int m, bh_int n, bh_int kl, bh_int ku, double *AB, bh_int ldab,
+              bh_int *ipiv, bh_int *info_host);
+int bh_sgbtrf(bh_int m, bh_int n, bh_int kl, bh_int ku, float *AB, bh_int ldab,
+              bh_int *ipiv, bh_int *info_host);
+
+/* ------------------------------------------------------------- gbtrs */
+/* Solve op(A) X = B with the factors of bh_?gbtrf; B is n x nrhs, overwritten.
+ * 'N' equals LAPACK dgbtrs bit for bit.
+ * Replaces LAPACK.gbtrs! in ldiv!(::BandedLU, B) (src/banded/linalg.jl:24-30)
+ * and its transposed form (src/banded/linalg.jl:41-47,55-56). */
+int bh_dgbtrs(char trans, bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const double *AB,
+              bh_int ldab, const bh_int *ipiv, double *B, bh_int ldb);
+int bh_sgbtrs(char trans, bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const float *AB,
+              bh_int ldab, const bh_int *ipiv, float *B, bh_int ldb);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BANDEDHIP_H */

@@ -1,0 +1,411 @@
+/*
+ * bandoracle.c -- CPU oracle for the banded hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: it is compiled into
+ * oracle/liboracle.so and loaded by tests/, by __graft_entry__.smoke() and by
+ * the cpu_baseline leg of bench.py.  Nothing in the product
+ * (bandedmatrices.jl_amd/) links, loads or calls it.
+ *
+ * It restates, in plain C, the arithmetic that BandedMatrices.jl reaches on its
+ * hot path:
+ *   - OpenBLAS ?gbmv 'N'/'T'        (src/generic/matmul.jl:21-23, src/blas.jl:16-28)
+ *   - gbmm! column decomposition    (src/banded/gbmm.jl:18-158,207-340)
+ *   - LAPACK ?gbtf2 / ?gbtrf        (src/banded/BandedLU.jl:90-103)
+ *   - LAPACK ?gbtrs 'N'/'T'         (src/banded/linalg.jl:24-47)
+ *   - BLAS ?tbsv upper              (src/tribanded.jl:75-83, src/blas.jl:123-141)
+ *   - banded Householder QR         (src/banded/bandedqr.jl:8-35) with the
+ *     reflector!/reflectorApply! pair it imports from ArrayLayouts
+ *   - Q / Q' application and solve  (src/banded/bandedqr.jl:38-94,175-206)
+ *
+ * Band storage (docs/src/index.md:236-252, BandedMatrix.jl:414-415): entry
+ * (i,j), 0-based, of a matrix with bandwidths (kl,ku) lives at
+ * ab[(ku + i - j) + j*ldab] when -kl <= j-i <= ku.
+ *
+ * Rounding recipes (which operations are fused) follow black-box probes of
+ * OpenBLAS 0.3.28: gbmv 'N' is y_i = fma(alpha*x_j, a_ij, y_i) with j
+ * ascending; gbtf2 scales by a rounded reciprocal and updates with
+ * fma(l_i, -u_jc, a_ic); gbtrs 'N' forward is fma(l, -b_j, b) and the back
+ * substitution divides.  This file is built with -ffp-contract=off so that
+ * only the explicit fma() calls are fused.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef int64_t i64;
+
+#define IDX(ld, r, c) ((r) + (c) * (ld))
+static i64 imax(i64 a, i64 b) { return a > b ? a : b; }
+static i64 imin(i64 a, i64 b) { return a < b ? a : b; }
+
+/* ------------------------------------------------------------------ gbmv */
+/* y <- alpha*op(A)*x + beta*y, A m-by-n with bands (kl,ku) in band storage.
+ * beta == 0 overwrites y (NaN-safe), as matmul.jl:26-39 / utils.jl:29-31
+ * require.  'N' follows the column-ordered OpenBLAS kernel; 'T' is a
+ * per-column dot product accumulated in ascending row order. */
+#define GBMV_BODY(T, FMA)                                                        \
+    if (m <= 0 || n <= 0) return;                                                \
+    i64 leny = (trans == 'N' || trans == 'n') ? m : n;                           \
+    i64 lenx = (trans == 'N' || trans == 'n') ? n : m;                           \
+    for (i64 i = 0; i < leny; ++i) {                                             \
+        T *yi = y + i * incy;                                                    \
+        *yi = (beta == (T)0) ? (T)0 : beta * *yi;                                \
+    }                                                                            \
+    if (trans == 'N' || trans == 'n') {                                          \
+        for (i64 j = 0; j < lenx; ++j) {                                         \
+            T t = alpha * x[j * incx];                                           \
+            i64 lo = imax(0, j - ku), hi = imin(m - 1, j + kl);                  \
+            for (i64 i = lo; i <= hi; ++i)                                       \
+                y[i * incy] = FMA(t, a[IDX(lda, ku + i - j, j)], y[i * incy]);   \
+        }                                                                        \
+    } else {                                                                     \
+        for (i64 j = 0; j < leny; ++j) {                                         \
+            T acc = 0;                                                           \
+            i64 lo = imax(0, j - ku), hi = imin(m - 1, j + kl);                  \
+            for (i64 i = lo; i <= hi; ++i)                                       \
+                acc = FMA(a[IDX(lda, ku + i - j, j)], x[i * incx], acc);         \
+            y[j * incy] = FMA(alpha, acc, y[j * incy]);                          \
+        }                                                                        \
+    }
+
+void bo_dgbmv(char trans, i64 m, i64 n, i64 kl, i64 ku, double alpha,
+              const double *a, i64 lda, const double *x, i64 incx, double beta,
+              double *y, i64 incy) {
+    GBMV_BODY(double, fma)
+}
+void bo_sgbmv(char trans, i64 m, i64 n, i64 kl, i64 ku, float alpha,
+              const float *a, i64 lda, const float *x, i64 incx, float beta,
+              float *y, i64 incy) {
+    GBMV_BODY(float, fmaf)
+}
+
+/* --------------------------------------------------------------- gbmm! */
+/* C <- alpha*A*B + beta*C for banded A (n x nu, bands Al,Au >= 0),
+ * B (nu x m, Bl,Bu >= 0), C (n x m, Cl,Cu) -- the positive-band core of
+ * gbmm! (gbmm.jl:282-340).  Every column j of C is one gbmv over the
+ * contraction index nu ascending (gbmm.jl:306-336), so an entry is
+ *   c_ij = fma(alpha*b_vj, a_iv, c_ij)   for v ascending,
+ * seeded with beta*c_ij (or 0 when beta == 0).  Rows of C's band outside the
+ * product's band and columns the product does not reach are beta-scaled
+ * (gbmm.jl:286-289,339).  Returns 0, or -1 for BandError (C too narrow and
+ * the outer bands of A,B are not exactly zero, gbmm.jl:250-279). */
+static i64 zeroband_u(const double *a, i64 lda, i64 rows, i64 cols, i64 l, i64 u) {
+    /* _num_zeroband_u (gbmm.jl:191-197): leading all-zero super bands */
+    for (i64 b = 0; b <= l + u; ++b) {
+        i64 k = u - b; /* band index: j - i = k */
+        for (i64 j = imax(0, k); j < cols && j - k < rows; ++j)
+            if (a[IDX(lda, u - k, j)] != 0.0) return b;
+    }
+    return l + u + 1;
+}
+static i64 zeroband_l(const double *a, i64 lda, i64 rows, i64 cols, i64 l, i64 u) {
+    /* _num_zeroband_l (gbmm.jl:199-205): trailing all-zero sub bands */
+    for (i64 b = 0; b <= l + u; ++b) {
+        i64 k = -l + b;
+        for (i64 j = imax(0, k); j < cols && j - k < rows; ++j)
+            if (a[IDX(lda, u - k, j)] != 0.0) return b;
+    }
+    return l + u + 1;
+}
+
+static void scale_or_zero(double beta, double *p) { *p = (beta == 0.0) ? 0.0 : beta * *p; }
+
+int bo_dgbmm(i64 n, i64 nu, i64 m, double alpha,
+             const double *A, i64 lda, i64 Al, i64 Au,
+             const double *B, i64 ldb, i64 Bl, i64 Bu,
+             double beta, double *C, i64 ldc, i64 Ctl, i64 Ctu) {
+    if (n == 0 || m == 0) return 0;
+    /* product bands clipped to the size of C (gbmm.jl:228, broadcast.jl:895) */
+    i64 Cl = imin(n - 1, Al + Bl), Cu = imin(m - 1, Au + Bu);
+    if (Ctu < Cu) {
+        i64 Aur = zeroband_u(A, lda, n, nu, Al, Au), Bur = zeroband_u(B, ldb, nu, m, Bl, Bu);
+        if (Ctu < Cu - Aur - Bur) return -1;
+        if (Au - Aur < -Al || Bu - Bur < -Bl) {
+            for (i64 j = 0; j < m; ++j)
+                for (i64 r = 0; r < Ctl + Ctu + 1; ++r) scale_or_zero(beta, &C[IDX(ldc, r, j)]);
+            return 0;
+        }
+        return bo_dgbmm(n, nu, m, alpha, A + Aur, lda, Al, Au - Aur, B + Bur, ldb, Bl, Bu - Bur,
+                        beta, C, ldc, Ctl, Ctu);
+    }
+    if (Ctl < Cl) {
+        i64 Alr = zeroband_l(A, lda, n, nu, Al, Au), Blr = zeroband_l(B, ldb, nu, m, Bl, Bu);
+        if (Ctl < Cl - Alr - Blr) return -1;
+        if (Al - Alr < -Au || Bl - Blr < -Bu) {
+            for (i64 j = 0; j < m; ++j)
+                for (i64 r = 0; r < Ctl + Ctu + 1; ++r) scale_or_zero(beta, &C[IDX(ldc, r, j)]);
+            return 0;
+        }
+        return bo_dgbmm(n, nu, m, alpha, A, lda, Al - Alr, Au, B, ldb, Bl - Blr, Bu,
+                        beta, C, ldc, Ctl, Ctu);
+    }
+    /* extra bands of C outside (Cl,Cu) are beta-scaled (gbmm.jl:286-288) */
+    for (i64 j = 0; j < m; ++j) {
+        for (i64 r = 0; r < Ctu - Cu; ++r) scale_or_zero(beta, &C[IDX(ldc, r, j)]);
+        for (i64 r = Ctu + Cl + 1; r < Ctl + Ctu + 1; ++r) scale_or_zero(beta, &C[IDX(ldc, r, j)]);
+    }
+    double *Cd = C + (Ctu - Cu); /* shifted view with bands (Cl,Cu) */
+    for (i64 j = 0; j < m; ++j) {
+        /* rows of column j that the product (A*B)[:,j] covers */
+        if (j >= nu + Bu) { /* untouched columns (gbmm.jl:339) */
+            if (j < imin(m, n + Cu))
+                for (i64 r = 0; r < Cl + Cu + 1; ++r) scale_or_zero(beta, &Cd[IDX(ldc, r, j)]);
+            continue;
+        }
+        i64 ilo = imax(0, j - Cu), ihi = imin(n - 1, j + Cl);
+        for (i64 i = ilo; i <= ihi; ++i) {
+            double *c = &Cd[IDX(ldc, Cu + i - j, j)];
+            double acc = (beta == 0.0) ? 0.0 : beta * *c;
+            i64 vlo = imax(imax(0, j - Bu), i - Al), vhi = imin(imin(nu - 1, j + Bl), i + Au);
+            for (i64 v = vlo; v <= vhi; ++v) {
+                double t = alpha * B[IDX(ldb, Bu + v - j, j)];
+                acc = fma(t, A[IDX(lda, Au + i - v, v)], acc);
+            }
+            *c = acc;
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------- gbtf2 */
+/* Unblocked partial-pivot band LU, restating LAPACK xGBTF2 (the routine
+ * xGBTRF runs for ku <= 64).  ab is (2kl+ku+1) x n, diagonal at row kl+ku.
+ * ipiv is 1-based (LAPACK convention, BandedLU.jl:12).  Returns info. */
+#define GBTF2_BODY(T, FMA, ABS)                                                  \
+    i64 kv = ku + kl, info = 0;                                                  \
+    if (m == 0 || n == 0) return 0;                                              \
+    for (i64 j = ku + 1; j < imin(kv, n); ++j)                                   \
+        for (i64 i = kv - j; i < kl; ++i) ab[IDX(ldab, i, j)] = 0;               \
+    i64 ju = 0;                                                                  \
+    for (i64 j = 0; j < imin(m, n); ++j) {                                       \
+        if (j + kv < n)                                                          \
+            for (i64 i = 0; i < kl; ++i) ab[IDX(ldab, i, j + kv)] = 0;           \
+        i64 km = imin(kl, m - 1 - j);                                            \
+        i64 jp = 0;                                                              \
+        T best = ABS(ab[IDX(ldab, kv, j)]);                                      \
+        for (i64 t = 1; t <= km; ++t) {                                          \
+            T v = ABS(ab[IDX(ldab, kv + t, j)]);                                 \
+            if (v > best) { best = v; jp = t; }                                  \
+        }                                                                        \
+        ipiv[j] = jp + j + 1;                                                    \
+        if (ab[IDX(ldab, kv + jp, j)] != 0) {                                    \
+            ju = imax(ju, imin(j + ku + jp, n - 1));                             \
+            if (jp != 0)                                                         \
+                for (i64 c = j; c <= ju; ++c) {                                  \
+                    T *p = &ab[IDX(ldab, kv + jp + j - c, c)];                   \
+                    T *q = &ab[IDX(ldab, kv + j - c, c)];                        \
+                    T s = *p; *p = *q; *q = s;                                   \
+                }                                                                \
+            if (km > 0) {                                                        \
+                T r = (T)1 / ab[IDX(ldab, kv, j)];                               \
+                for (i64 t = 1; t <= km; ++t) {                                  \
+                    if (variant == 1) ab[IDX(ldab, kv + t, j)] /= ab[IDX(ldab, kv, j)]; \
+                    else ab[IDX(ldab, kv + t, j)] *= r;                          \
+                }                                                                \
+                for (i64 c = j + 1; c <= ju; ++c) {                              \
+                    T nu_ = -ab[IDX(ldab, kv + j - c, c)];                       \
+                    for (i64 t = 1; t <= km; ++t) {                              \
+                        T *q_ = &ab[IDX(ldab, kv + j + t - c, c)];               \
+                        if (variant == 2) *q_ = *q_ + ab[IDX(ldab, kv + t, j)] * nu_; \
+                        else *q_ = FMA(ab[IDX(ldab, kv + t, j)], nu_, *q_);      \
+                    }                                                            \
+                }                                                                \
+            }                                                                    \
+        } else if (info == 0) {                                                  \
+            info = j + 1;                                                        \
+        }                                                                        \
+    }                                                                            \
+    return info;
+
+i64 bo_dgbtf2(i64 m, i64 n, i64 kl, i64 ku, double *ab, i64 ldab, i64 *ipiv) {
+    const int variant = 0;
+    GBTF2_BODY(double, fma, fabs)
+}
+i64 bo_sgbtf2(i64 m, i64 n, i64 kl, i64 ku, float *ab, i64 ldab, i64 *ipiv) {
+    const int variant = 0;
+    GBTF2_BODY(float, fmaf, fabsf)
+}
+/* Planted mistakes for the pin tests only: variant 1 divides by the pivot
+ * instead of scaling by its reciprocal, variant 2 does not fuse the update.
+ * Each must fail the bit pin against LAPACK. */
+i64 bo_dgbtf2_planted(int variant, i64 m, i64 n, i64 kl, i64 ku, double *ab, i64 ldab, i64 *ipiv) {
+    GBTF2_BODY(double, fma, fabs)
+}
+
+/* ------------------------------------------------------------- tbsv */
+/* Upper-triangular band solve, x <- U^{-1} x ('N') or U^{-T} x ('T'),
+ * U n x n with k super-diagonals stored with the diagonal at row k of a.
+ * 'N' restates the column-oriented BLAS xTBSV: x_j = b_j / u_jj, then
+ * b_i = fma(u_ij, -x_j, b_i).  'T' accumulates each dot in ascending order. */
+#define TBSV_BODY(T, FMA)                                                        \
+    if (trans == 'N' || trans == 'n') {                                          \
+        for (i64 j = n - 1; j >= 0; --j) {                                       \
+            T xj = x[j * incx] / a[IDX(lda, k, j)];                              \
+            x[j * incx] = xj;                                                    \
+            T nx = -xj;                                                          \
+            for (i64 i = imax(0, j - k); i < j; ++i)                             \
+                x[i * incx] = FMA(a[IDX(lda, k + i - j, j)], nx, x[i * incx]);   \
+        }                                                                        \
+    } else {                                                                     \
+        for (i64 j = 0; j < n; ++j) {                                            \
+            T acc = x[j * incx];                                                 \
+            for (i64 i = imax(0, j - k); i < j; ++i)                             \
+                acc = FMA(-a[IDX(lda, k + i - j, j)], x[i * incx], acc);         \
+            x[j * incx] = acc / a[IDX(lda, k, j)];                               \
+        }                                                                        \
+    }
+
+void bo_dtbsv_u(char trans, i64 n, i64 k, const double *a, i64 lda, double *x, i64 incx) {
+    TBSV_BODY(double, fma)
+}
+void bo_stbsv_u(char trans, i64 n, i64 k, const float *a, i64 lda, float *x, i64 incx) {
+    TBSV_BODY(float, fmaf)
+}
+
+/* ------------------------------------------------------------- gbtrs */
+/* Solve with the factors of gbtf2/gbtrf, restating LAPACK xGBTRS.
+ * 'N': forward sweep with the stored interchanges and multipliers, then the
+ * upper band solve with kl+ku super-diagonals.  'T': transposed upper solve,
+ * then the transposed unit-lower sweep in reverse with the interchanges. */
+#define GBTRS_BODY(T, FMA, TBSV)                                                 \
+    i64 kd = ku + kl;                                                            \
+    if (n == 0 || nrhs == 0) return;                                             \
+    if (trans == 'N' || trans == 'n') {                                          \
+        if (kl > 0)                                                              \
+            for (i64 j = 0; j < n - 1; ++j) {                                    \
+                i64 lm = imin(kl, n - 1 - j), p = ipiv[j] - 1;                   \
+                for (i64 r = 0; r < nrhs; ++r) {                                 \
+                    T *bc = b + r * ldb;                                         \
+                    if (p != j) { T s = bc[p]; bc[p] = bc[j]; bc[j] = s; }       \
+                    T nb = -bc[j];                                               \
+                    for (i64 t = 1; t <= lm; ++t)                                \
+                        bc[j + t] = FMA(ab[IDX(ldab, kd + t, j)], nb, bc[j + t]);\
+                }                                                                \
+            }                                                                    \
+        for (i64 r = 0; r < nrhs; ++r) TBSV('N', n, kd, ab, ldab, b + r * ldb, 1);\
+    } else {                                                                     \
+        for (i64 r = 0; r < nrhs; ++r) TBSV('T', n, kd, ab, ldab, b + r * ldb, 1);\
+        if (kl > 0)                                                              \
+            for (i64 j = n - 2; j >= 0; --j) {                                   \
+                i64 lm = imin(kl, n - 1 - j), p = ipiv[j] - 1;                   \
+                for (i64 r = 0; r < nrhs; ++r) {                                 \
+                    T *bc = b + r * ldb;                                         \
+                    T acc = 0;                                                   \
+                    for (i64 t = 1; t <= lm; ++t)                                \
+                        acc = FMA(bc[j + t], ab[IDX(ldab, kd + t, j)], acc);     \
+                    bc[j] = bc[j] - acc;                                         \
+                    if (p != j) { T s = bc[p]; bc[p] = bc[j]; bc[j] = s; }       \
+                }                                                                \
+            }                                                                    \
+    }
+
+void bo_dgbtrs(char trans, i64 n, i64 kl, i64 ku, i64 nrhs, const double *ab, i64 ldab,
+               const i64 *ipiv, double *b, i64 ldb) {
+    GBTRS_BODY(double, fma, bo_dtbsv_u)
+}
+void bo_sgbtrs(char trans, i64 n, i64 kl, i64 ku, i64 nrhs, const float *ab, i64 ldab,
+               const i64 *ipiv, float *b, i64 ldb) {
+    GBTRS_BODY(float, fmaf, bo_stbsv_u)
+}
+
+/* -------------------------------------------------------- banded QR */
+/* reflector! (ArrayLayouts, imported at src/BandedMatrices.jl:28-35):
+ * x <- [beta; v(2:end)], returns tau with H = I - tau*[1;v][1;v]'.
+ * beta = -copysign(||x||, x1); tau = (x1 + copysign(||x||,x1)) / copysign(||x||,x1).
+ * ||x|| == 0 returns tau = 0 and leaves x.  When x(2:end) == 0 and x1 != 0
+ * this gives tau = 2 (LAPACK's dlarfg would return 0 there). */
+static double reflector(double *x, i64 len) {
+    if (len == 0) return 0.0;
+    double xi1 = x[0];
+    double nrm = xi1 * xi1;
+    for (i64 i = 1; i < len; ++i) nrm += x[i] * x[i];
+    if (nrm == 0.0) return 0.0;
+    nrm = sqrt(nrm);
+    double nu = copysign(nrm, xi1);
+    xi1 += nu;
+    x[0] = -nu;
+    for (i64 i = 1; i < len; ++i) x[i] /= xi1;
+    return xi1 / nu;
+}
+/* reflectorApply!: y <- (I - tau [1;v][1;v]') y for one column slice y. */
+static void reflector_apply(const double *x, i64 len, double tau, double *y) {
+    if (len == 0) return;
+    double v = y[0];
+    for (i64 i = 1; i < len; ++i) v += x[i] * y[i];
+    v = tau * v;
+    y[0] -= v;
+    for (i64 i = 1; i < len; ++i) y[i] -= x[i] * v;
+}
+
+/* _banded_qr! (bandedqr.jl:8-28) on the padded copy: R is m x n with bands
+ * (l, u) where u already includes the l fill-in super-diagonals
+ * (bandedqr.jl:2).  tau has min(m,n) entries; entries past ncols stay 0. */
+void bo_dgbqrf(i64 m, i64 n, i64 l, i64 u, double *D, i64 ldd, double *tau) {
+    i64 nu = l + u + 1;
+    i64 ncols = imin(m - 1, n);
+    for (i64 k = 0; k < imin(m, n); ++k) tau[k] = 0.0;
+    for (i64 k = 0; k < ncols; ++k) {
+        i64 len = imin(nu, m - k + u) - u; /* rows u..min(nu,m-k+u)-1 */
+        double *x = &D[IDX(ldd, u, k)];
+        double tk = reflector(x, len);
+        tau[k] = tk;
+        for (i64 j = 1; j <= imin(u, n - 1 - k); ++j) {
+            double *y = &D[IDX(ldd, u - j, k + j)];
+            reflector_apply(x, len, tk, y);
+        }
+    }
+}
+
+/* banded_qr_lmul! for Q' (bandedqr.jl:66-94): B <- Q' B, B is m x nrhs dense. */
+void bo_dgbqr_qtmul(i64 m, i64 n, i64 l, i64 u, const double *D, i64 ldd, const double *tau,
+                    i64 ntau, i64 nrhs, double *B, i64 ldb) {
+    for (i64 j = 0; j < nrhs; ++j) {
+        double *bj = B + j * ldb;
+        i64 kmax = imin(imin(m, n), ntau);
+        for (i64 k = 0; k < kmax; ++k) {
+            double v = bj[k];
+            i64 ilo = k + imax(1, -u), ihi = imin(k + l, m - 1);
+            for (i64 i = ilo; i <= ihi; ++i) v += D[IDX(ldd, i - k + u, k)] * bj[i];
+            v = tau[k] * v;
+            bj[k] -= v;
+            for (i64 i = ilo; i <= ihi; ++i) bj[i] -= D[IDX(ldd, i - k + u, k)] * v;
+        }
+    }
+}
+
+/* banded_qr_lmul! for Q (bandedqr.jl:38-64): B <- Q B. */
+void bo_dgbqr_qmul(i64 m, i64 n, i64 l, i64 u, const double *D, i64 ldd, const double *tau,
+                   i64 ntau, i64 nrhs, double *B, i64 ldb) {
+    i64 kmax = imin(imin(m, n), ntau);
+    for (i64 k = kmax - 1; k >= 0; --k) {
+        for (i64 j = 0; j < nrhs; ++j) {
+            double *bj = B + j * ldb;
+            double v = bj[k];
+            i64 ilo = k + imax(1, -u), ihi = imin(k + l, m - 1);
+            for (i64 i = ilo; i <= ihi; ++i) v += D[IDX(ldd, i - k + u, k)] * bj[i];
+            v = tau[k] * v;
+            bj[k] -= v;
+            for (i64 i = ilo; i <= ihi; ++i) bj[i] -= D[IDX(ldd, i - k + u, k)] * v;
+        }
+    }
+}
+
+/* ldiv!(QR, B) for m >= n (bandedqr.jl:178-191): B <- Q'B, then
+ * R[1:n,1:n] \ B[1:n,:] with the tbsv upper solve of bandwidth u. */
+void bo_dgbqrs(i64 m, i64 n, i64 l, i64 u, const double *D, i64 ldd, const double *tau,
+               i64 nrhs, double *B, i64 ldb) {
+    bo_dgbqr_qtmul(m, n, l, u, D, ldd, tau, imin(m, n), nrhs, B, ldb);
+    for (i64 r = 0; r < nrhs; ++r) bo_dtbsv_u('N', n, u, D, ldd, B + r * ldb, 1);
+}
+
+/* ------------------------------------------------ adjoint of A\b + 1/2|Ax0|^2 */
+/* Gradient of L = w' A^{-1} b + 0.5*||A x0||^2 w.r.t. the band entries of A
+ * and b, given x = A\b, lambda = A^{-T} w and y0 = A x0:
+ *   dL/db = lambda,   dL/dA_ij = -lambda_i x_j + y0_i x0_j   (in band). */
+void bo_dgb_band_grad(i64 n, i64 kl, i64 ku, const double *lam, const double *x,
+                      const double *y0, const double *x0, double *G, i64 ldg) {
+    for (i64 j = 0; j < n; ++j)
+        for (i64 i = imax(0, j - ku); i <= imin(n - 1, j + kl); ++i)
+            G[IDX(ldg, ku + i - j, j)] = fma(y0[i], x0[j], -lam[i] * x[j]);
+}
