@@ -10,9 +10,11 @@ from .device import DeviceArray, Timer, sync
 from .bandedmatrix import (BandedMatrix, BandError, DimensionMismatch, _BandedMatrix, brand, gbmm, mul,
                            prodbandwidths)
 from .lu import BandedLU, SingularException, TransposeFact, ldiv_, lu, lu_, pad_for_lu, solve
+from .mixed import MixedResult, MixedSolver, solve_mixed
 
 __all__ = [
     "BandedHIPError", "load_library", "LIB_PATH", "DeviceArray", "Timer", "sync", "BandedMatrix", "BandError",
     "DimensionMismatch", "_BandedMatrix", "brand", "gbmm", "mul", "prodbandwidths", "BandedLU",
-    "SingularException", "TransposeFact", "ldiv_", "lu", "lu_", "pad_for_lu", "solve",
+    "SingularException", "TransposeFact", "ldiv_", "lu", "lu_", "pad_for_lu", "solve", "MixedResult",
+    "MixedSolver", "solve_mixed",
 ]

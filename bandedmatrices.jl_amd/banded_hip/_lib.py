@@ -47,6 +47,8 @@ for _p, _t in (("d", dbl), ("s", flt)):
     SIGNATURES[f"bh_{_p}gbtrs"] = (C.c_int, [ch, i64, i64, i64, i64, vp, i64, vp, vp, i64])
 
 SIGNATURES["bh_dsgbpad"] = (C.c_int, [i64, i64, i64, i64, vp, i64, vp, i64])
+SIGNATURES["bh_dsgbsv_workspace"] = (C.c_int, [i64, i64, i64, i64, C.POINTER(C.c_size_t)])
+SIGNATURES["bh_dsgbsv"] = (C.c_int, [i64, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp, C.c_size_t, vp, pi64, pi64])
 
 _lib = None
 

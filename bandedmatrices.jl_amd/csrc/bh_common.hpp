@@ -53,6 +53,16 @@ __host__ __device__ __forceinline__ i64 cdiv(i64 a, i64 b) { return (a + b - 1) 
 __device__ __forceinline__ double dfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float dfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
+// Workgroup barrier that orders LDS only: it waits for this wave's LDS
+// traffic, not for outstanding global loads/stores (which __syncthreads()
+// would, stalling every step on HBM latency).  Use it where no thread reads
+// global data another thread of the workgroup wrote.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ double dabs(double x) { return __builtin_fabs(x); }
 __device__ __forceinline__ float dabs(float x) { return __builtin_fabsf(x); }
 

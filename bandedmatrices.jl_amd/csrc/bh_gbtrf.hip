@@ -129,6 +129,9 @@ __global__ void __launch_bounds__(kLuThreads) gbtf2_single(LuSys<T> s, i64 *info
 }
 
 template <typename T>
+int reg_gbtrf_launch(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, i64 *dinfo, hipStream_t st);
+
+template <typename T>
 static int gbtrf_impl(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, i64 *info_host) {
     BH_ARG(m >= 0, 1);
     BH_ARG(n >= 0, 2);
@@ -140,9 +143,16 @@ static int gbtrf_impl(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, 
     if (m == 0 || n == 0) return BH_OK;
     i64 *dinfo = nullptr;
     BH_HIP(hipMallocAsync((void **)&dinfo, sizeof(i64), stream()));
-    LuSys<T> s{ab, ipiv, m, n, kl, ku, ldab};
-    hipLaunchKernelGGL(gbtf2_single<T>, dim3(1), dim3(kLuThreads), 0, stream(), s, dinfo);
-    BH_LAUNCH_CHECK();
+    // register-resident kernel when the window fits one of its shapes,
+    // otherwise the barrier-stepped kernel working in global memory
+    int st = reg_gbtrf_launch<T>(m, n, kl, ku, ab, ldab, ipiv, dinfo, stream());
+    if (st == -1) {
+        LuSys<T> s{ab, ipiv, m, n, kl, ku, ldab};
+        hipLaunchKernelGGL(gbtf2_single<T>, dim3(1), dim3(kLuThreads), 0, stream(), s, dinfo);
+        BH_LAUNCH_CHECK();
+    } else if (st != BH_OK) {
+        return st;
+    }
     BH_HIP(hipMemcpyAsync(info_host, dinfo, sizeof(i64), hipMemcpyDeviceToHost, stream()));
     BH_HIP(hipFreeAsync(dinfo, stream()));
     BH_HIP(hipStreamSynchronize(stream()));

@@ -98,6 +98,10 @@ gbtrs_kernel(int trans_n, i64 n, i64 kl, i64 ku, const T *ab, i64 ld, const i64 
 }
 
 template <typename T>
+int wave_gbtrs_launch(bool trans_n, i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 ld, const i64 *ipiv, T *B,
+                      i64 ldb, hipStream_t st);
+
+template <typename T>
 static int gbtrs_impl(char trans, i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 ldab, const i64 *ipiv, T *B,
                       i64 ldb) {
     BH_ARG(trans == 'N' || trans == 'n' || trans == 'T' || trans == 't' || trans == 'C' || trans == 'c', 1);
@@ -109,6 +113,10 @@ static int gbtrs_impl(char trans, i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, 
     BH_ARG(ldb >= imax(1, n), 10);
     if (n == 0 || nrhs == 0) return BH_OK;
     const int tn = (trans == 'N' || trans == 'n');
+    // one wave per right-hand side with a register window when the band fits,
+    // otherwise the barrier-stepped kernel working in global memory
+    const int st = wave_gbtrs_launch<T>(tn != 0, n, kl, ku, nrhs, ab, ldab, ipiv, B, ldb, stream());
+    if (st != -1) return st;
     hipLaunchKernelGGL(gbtrs_kernel<T>, dim3((unsigned)nrhs), dim3(kTrsThreads), 0, stream(), tn, n, kl, ku, ab, ldab,
                        ipiv, B, ldb);
     BH_LAUNCH_CHECK();

@@ -1,0 +1,242 @@
+// bh_mixed.hip -- FP32 band LU with FP64 iterative refinement (no reference
+// counterpart: the reference only promotes eltypes, src/banded/BandedLU.jl:110).
+//
+// x = A \ B for an FP64 band matrix:
+//   1. AB32 = padded FP32 copy of A; FP32 partial-pivot band LU (bh_sgbtrf);
+//   2. x0 = FP32 solve of B (bh_sgbtrs), widened to FP64;
+//   3. repeat: r = B - A x in FP64 (bh_dgbmv_multi, A read once for all
+//      right-hand sides); stop when every column satisfies LAPACK dsgesv's
+//      rule ||r||_inf <= sqrt(n) * eps64 * ||A||_inf * ||x||_inf; else
+//      d = FP32 solve of r, x += d (FP64 accumulation);
+//   4. after 30 iterations without convergence (or an FP32 overflow or a zero
+//      FP32 pivot) fall back to FP64 LU + solve, reported as iters < 0.
+// Residuals, norms and the solution are FP64 throughout.
+#include <math.h>
+
+#include <vector>
+
+#include "bh_common.hpp"
+
+namespace bh {
+
+__global__ void d2s_kernel(i64 rows, i64 cols, const double *__restrict__ src, i64 lds, float *__restrict__ dst,
+                           i64 ldd) {
+    const i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= rows * cols) return;
+    const i64 c = e / rows, r = e - c * rows;
+    dst[c * ldd + r] = (float)src[c * lds + r];
+}
+
+__global__ void s2d_add_kernel(i64 rows, i64 cols, const float *__restrict__ d, i64 ldd, double *__restrict__ x,
+                               i64 ldx, int assign) {
+    const i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= rows * cols) return;
+    const i64 c = e / rows, r = e - c * rows;
+    const double v = (double)d[c * ldd + r];
+    x[c * ldx + r] = assign ? v : x[c * ldx + r] + v;
+}
+
+// Non-negative doubles order like their bit patterns, so a max over
+// magnitudes can use the 64-bit unsigned atomic max.
+__device__ __forceinline__ void atomic_max_nonneg(double *addr, double v) {
+    atomicMax(reinterpret_cast<unsigned long long *>(addr), (unsigned long long)__double_as_longlong(v));
+}
+
+// out[c] = max_r |v[r,c]| (out must start at 0)
+__global__ void colmax_kernel(i64 rows, i64 cols, const double *__restrict__ v, i64 ld, double *out) {
+    const i64 c = blockIdx.y;
+    double m = 0.0;
+    for (i64 r = (i64)blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += (i64)gridDim.x * blockDim.x) {
+        const double a = fabs(v[c * ld + r]);
+        m = (a > m || a != a) ? a : m;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m, off, 64);
+        m = (o > m || o != o) ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&out[c], m);
+}
+
+// out = max_i sum_j |A(i,j)| over the band (||A||_inf)
+__global__ void band_inf_norm_kernel(i64 n, i64 kl, i64 ku, const double *__restrict__ A, i64 lda, double *out) {
+    const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    if (i < n) {
+        for (i64 j = imax(0, i - kl); j <= imin(n - 1, i + ku); ++j) s += fabs(A[j * lda + ku + i - j]);
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(s, off, 64);
+        s = (o > s || o != o) ? o : s;
+    }
+    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(out, s);
+}
+
+__global__ void fill_zero_kernel(double *p, i64 cnt) {
+    const i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < cnt) p[e] = 0.0;
+}
+
+static inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+struct MixedWs {
+    float *ab32;
+    i64 *ipiv;
+    i64 *info;
+    double *r;
+    float *r32;
+    double *norms;  // [0] = ||A||, [1..nrhs] = ||x_c||, [1+nrhs..] = ||r_c||
+    size_t bytes;
+};
+
+static MixedWs mixed_layout(i64 n, i64 kl, i64 ku, i64 nrhs, void *base) {
+    MixedWs w{};
+    char *p = (char *)base;
+    size_t off = 0;
+    const i64 ldab = 2 * kl + ku + 1;
+    w.ab32 = (float *)(p + off);
+    off += align256((size_t)ldab * n * sizeof(float));
+    w.ipiv = (i64 *)(p + off);
+    off += align256((size_t)imax(n, 1) * sizeof(i64));
+    w.info = (i64 *)(p + off);
+    off += align256(sizeof(i64));
+    w.r = (double *)(p + off);
+    off += align256((size_t)n * nrhs * sizeof(double));
+    w.r32 = (float *)(p + off);
+    off += align256((size_t)n * nrhs * sizeof(float));
+    w.norms = (double *)(p + off);
+    off += align256((size_t)(1 + 2 * nrhs) * sizeof(double));
+    w.bytes = off;
+    return w;
+}
+
+static dim3 grid1(i64 cnt) { return dim3((unsigned)cdiv(cnt, 256)); }
+
+}  // namespace bh
+
+using namespace bh;
+
+extern "C" {
+
+int bh_dsgbsv_workspace(bh_int n, bh_int kl, bh_int ku, bh_int nrhs, size_t *bytes) {
+    BH_ARG(n >= 0, 1);
+    BH_ARG(kl >= 0, 2);
+    BH_ARG(ku >= 0, 3);
+    BH_ARG(nrhs >= 0, 4);
+    BH_ARG(bytes != nullptr, 5);
+    *bytes = mixed_layout(n, kl, ku, nrhs, nullptr).bytes;
+    return BH_OK;
+}
+
+int bh_dsgbsv(bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const double *A, bh_int lda, const double *B, bh_int ldb,
+              double *X, bh_int ldx, void *work, size_t lwork, bh_int *ipiv32, bh_int *iters_host, bh_int *info_host) {
+    BH_ARG(n >= 0, 1);
+    BH_ARG(kl >= 0, 2);
+    BH_ARG(ku >= 0, 3);
+    BH_ARG(nrhs >= 0, 4);
+    BH_ARG(lda >= kl + ku + 1, 6);
+    BH_ARG(ldb >= imax(1, n), 8);
+    BH_ARG(ldx >= imax(1, n), 10);
+    BH_ARG(iters_host != nullptr, 14);
+    BH_ARG(info_host != nullptr, 15);
+    *iters_host = 0;
+    *info_host = 0;
+    if (n == 0 || nrhs == 0) return BH_OK;
+    MixedWs w = mixed_layout(n, kl, ku, nrhs, work);
+    if (work == nullptr || lwork < w.bytes) {
+        set_error("bh_dsgbsv: workspace of %zu bytes needed, %zu given", w.bytes, lwork);
+        return BH_ERR_WORKSPACE;
+    }
+    hipStream_t s = stream();
+    const i64 ldab = 2 * kl + ku + 1;
+    const double eps = 1.1102230246251565e-16;  // dlamch('Epsilon') = 2^-53
+    const int itmax = 30;
+    int st;
+
+    // ||A||_inf
+    hipLaunchKernelGGL(fill_zero_kernel, grid1(1 + 2 * nrhs), dim3(256), 0, s, w.norms, 1 + 2 * nrhs);
+    hipLaunchKernelGGL(band_inf_norm_kernel, grid1(n), dim3(256), 0, s, n, kl, ku, A, lda, w.norms);
+    BH_LAUNCH_CHECK();
+    // FP32 factorisation
+    st = bh_dsgbpad(n, n, kl, ku, A, lda, w.ab32, ldab);
+    if (st) return st;
+    i64 info32 = 0;
+    st = bh_sgbtrf(n, n, kl, ku, w.ab32, ldab, w.ipiv, &info32);
+    if (st) return st;
+    if (ipiv32) BH_HIP(hipMemcpyAsync(ipiv32, w.ipiv, n * sizeof(i64), hipMemcpyDeviceToDevice, s));
+    double anrm = 0.0;
+    BH_HIP(hipMemcpyAsync(&anrm, w.norms, sizeof(double), hipMemcpyDeviceToHost, s));
+    BH_HIP(hipStreamSynchronize(s));
+    const double cte = anrm * eps * sqrt((double)n);
+    bool fallback = (info32 != 0) || !(anrm <= 3.4028234663852886e38);
+    int iter = 0;
+    bool converged = false;
+    if (!fallback) {
+        // x0 = solve in FP32
+        hipLaunchKernelGGL(d2s_kernel, grid1(n * nrhs), dim3(256), 0, s, n, nrhs, B, ldb, w.r32, n);
+        BH_LAUNCH_CHECK();
+        st = bh_sgbtrs('N', n, kl, ku, nrhs, w.ab32, ldab, w.ipiv, w.r32, n);
+        if (st) return st;
+        hipLaunchKernelGGL(s2d_add_kernel, grid1(n * nrhs), dim3(256), 0, s, n, nrhs, w.r32, n, X, ldx, 1);
+        BH_LAUNCH_CHECK();
+        std::vector<double> nh(1 + 2 * nrhs);
+        for (iter = 0; iter <= itmax; ++iter) {
+            // r = B - A x (FP64)
+            BH_HIP(hipMemcpy2DAsync(w.r, n * sizeof(double), B, ldb * sizeof(double), n * sizeof(double), nrhs,
+                                    hipMemcpyDeviceToDevice, s));
+            st = bh_dgbmv_multi('N', n, n, kl, ku, nrhs, -1.0, A, lda, X, ldx, 1.0, w.r, n);
+            if (st) return st;
+            hipLaunchKernelGGL(fill_zero_kernel, grid1(2 * nrhs), dim3(256), 0, s, w.norms + 1, 2 * nrhs);
+            hipLaunchKernelGGL(colmax_kernel, dim3((unsigned)imin(cdiv(n, 256), 256), (unsigned)nrhs), dim3(256), 0, s,
+                               n, nrhs, X, ldx, w.norms + 1);
+            hipLaunchKernelGGL(colmax_kernel, dim3((unsigned)imin(cdiv(n, 256), 256), (unsigned)nrhs), dim3(256), 0, s,
+                               n, nrhs, w.r, n, w.norms + 1 + nrhs);
+            BH_LAUNCH_CHECK();
+            BH_HIP(hipMemcpyAsync(nh.data(), w.norms, (1 + 2 * nrhs) * sizeof(double), hipMemcpyDeviceToHost, s));
+            BH_HIP(hipStreamSynchronize(s));
+            bool ok = true;
+            for (i64 c = 0; c < nrhs; ++c) {
+                const double xn = nh[1 + c], rn = nh[1 + nrhs + c];
+                if (!(rn <= xn * cte)) ok = false;
+            }
+            if (ok) {
+                converged = true;
+                break;
+            }
+            if (iter == itmax) break;
+            // d = A32 \ r ; x += d
+            hipLaunchKernelGGL(d2s_kernel, grid1(n * nrhs), dim3(256), 0, s, n, nrhs, w.r, n, w.r32, n);
+            BH_LAUNCH_CHECK();
+            st = bh_sgbtrs('N', n, kl, ku, nrhs, w.ab32, ldab, w.ipiv, w.r32, n);
+            if (st) return st;
+            hipLaunchKernelGGL(s2d_add_kernel, grid1(n * nrhs), dim3(256), 0, s, n, nrhs, w.r32, n, X, ldx, 0);
+            BH_LAUNCH_CHECK();
+        }
+    }
+    if (converged) {
+        *iters_host = iter;
+        return BH_OK;
+    }
+    // FP64 fallback: LU + solve (allocates the FP64 factors)
+    double *ab64 = nullptr;
+    i64 *ipiv64 = nullptr;
+    BH_HIP(hipMallocAsync((void **)&ab64, (size_t)ldab * n * sizeof(double), s));
+    BH_HIP(hipMallocAsync((void **)&ipiv64, (size_t)n * sizeof(i64), s));
+    st = bh_dgbpad(n, n, kl, ku, A, lda, ab64, ldab);
+    i64 info64 = 0;
+    if (!st) st = bh_dgbtrf(n, n, kl, ku, ab64, ldab, ipiv64, &info64);
+    if (!st && info64 == 0) {
+        BH_HIP(hipMemcpy2DAsync(X, ldx * sizeof(double), B, ldb * sizeof(double), n * sizeof(double), nrhs,
+                                hipMemcpyDeviceToDevice, s));
+        st = bh_dgbtrs('N', n, kl, ku, nrhs, ab64, ldab, ipiv64, X, ldx);
+    }
+    BH_HIP(hipFreeAsync(ab64, s));
+    BH_HIP(hipFreeAsync(ipiv64, s));
+    BH_HIP(hipStreamSynchronize(s));
+    *info_host = info64;
+    *iters_host = -(iter + 1);
+    return st;
+}
+
+}  // extern "C"

@@ -133,6 +133,21 @@ int bh_dgbtrs(char trans, bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const dou
 int bh_sgbtrs(char trans, bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const float *AB,
               bh_int ldab, const bh_int *ipiv, float *B, bh_int ldb);
 
+/* ------------------------------------------------------- mixed precision */
+/* X = A \ B for an FP64 band matrix A ((kl+ku+1) x n, lda) via an FP32 band LU
+ * and FP64 iterative refinement: residuals B - A X in FP64, corrections from
+ * FP32 solves, stop when every column meets LAPACK dsgesv's test
+ * ||r||_inf <= sqrt(n) eps64 ||A||_inf ||x||_inf (at most 30 iterations), else
+ * fall back to an FP64 LU (then *iters_host < 0).  B is not modified.
+ * work: device workspace of bh_dsgbsv_workspace() bytes.  ipiv32 (device,
+ * optional) receives the FP32 factorisation's pivots.
+ * No reference counterpart (the reference factors in the input precision,
+ * src/banded/BandedLU.jl:108-111); it is a drop-in for A\b (linalg.jl:5-9). */
+int bh_dsgbsv_workspace(bh_int n, bh_int kl, bh_int ku, bh_int nrhs, size_t *bytes);
+int bh_dsgbsv(bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const double *A, bh_int lda,
+              const double *B, bh_int ldb, double *X, bh_int ldx, void *work, size_t lwork,
+              bh_int *ipiv32, bh_int *iters_host, bh_int *info_host);
+
 #ifdef __cplusplus
 }
 #endif
