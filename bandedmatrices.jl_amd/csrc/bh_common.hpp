@@ -63,6 +63,66 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// ---- cross-lane moves without LDS (DPP / readlane), for float and double.
+// DPP controls used (lane layout checked on gfx950):
+//   0x130 wave_shl1: lane i <- lane i+1      0x138 wave_shr1: lane i <- lane i-1
+//   0x111.. row_shr:k: lane i <- lane i-k within a 16-lane row
+//   0x142 row_bcast15 / 0x143 row_bcast31: last lane of a row to the next row(s)
+template <int CTRL, int ROWMASK = 0xF>
+__device__ __forceinline__ float dpp_mov(float old, float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, old),
+                                                                 __builtin_bit_cast(int, v), CTRL, ROWMASK, 0xF, false));
+}
+template <int CTRL, int ROWMASK = 0xF>
+__device__ __forceinline__ double dpp_mov(double old, double v) {
+    const long long o = __builtin_bit_cast(long long, old), x = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_update_dpp((int)o, (int)x, CTRL, ROWMASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(x >> 32), CTRL, ROWMASK, 0xF, false);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float lane_read(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ __forceinline__ double lane_read(double v, int l) {
+    const long long x = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)x, l), hi = __builtin_amdgcn_readlane((int)(x >> 32), l);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float lane_first(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double lane_first(double v) {
+    const long long x = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readfirstlane((int)x), hi = __builtin_amdgcn_readfirstlane((int)(x >> 32));
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+// v with lane l replaced by the (uniform) value x
+template <typename T>
+__device__ __forceinline__ T lane_write(T v, T x, int l) {
+    return ((int)__lane_id() == l) ? x : v;
+}
+// lane i <- lane i+1, lane 63 <- carry (uniform)
+template <typename T>
+__device__ __forceinline__ T wave_down1(T v, T carry) {
+    return lane_write(dpp_mov<0x130>(v, v), carry, 63);
+}
+// lane i <- lane i-1, lane 0 <- carry (uniform)
+template <typename T>
+__device__ __forceinline__ T wave_up1(T v, T carry) {
+    return lane_write(dpp_mov<0x138>(v, v), carry, 0);
+}
+// sum over the 64 lanes (result uniform); DPP partial sums, lane 63 holds the total
+template <typename T>
+__device__ __forceinline__ T wave_sum_dpp(T x) {
+    x += dpp_mov<0x111>(T(0), x);
+    x += dpp_mov<0x112>(T(0), x);
+    x += dpp_mov<0x114>(T(0), x);
+    x += dpp_mov<0x118>(T(0), x);
+    x += dpp_mov<0x142, 0xA>(T(0), x);
+    x += dpp_mov<0x143, 0xC>(T(0), x);
+    return lane_read(x, 63);
+}
+
 __device__ __forceinline__ double dabs(double x) { return __builtin_fabs(x); }
 __device__ __forceinline__ float dabs(float x) { return __builtin_fabsf(x); }
 

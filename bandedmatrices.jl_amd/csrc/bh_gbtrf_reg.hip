@@ -34,6 +34,64 @@ struct RegLuArgs {
     i64 m, n, kl, ku, ldab;
 };
 
+// Copy column register KK of every row slot (compile-time index after the
+// switch below, so no select chains).
+template <int KK, typename T, int RR, int CC>
+__device__ __forceinline__ void col_get(const T (&a)[RR][CC], T (&v)[RR]) {
+#pragma unroll
+    for (int i = 0; i < RR; ++i) v[i] = a[i][KK];
+}
+template <int KK, typename T, int RR, int CC>
+__device__ __forceinline__ void col_set(T (&a)[RR][CC], const T (&v)[RR]) {
+#pragma unroll
+    for (int i = 0; i < RR; ++i) a[i][KK] = v[i];
+}
+template <int I, typename T, int RR, int CC>
+__device__ __forceinline__ void row_get(const T (&a)[RR][CC], T (&v)[CC]) {
+#pragma unroll
+    for (int k = 0; k < CC; ++k) v[k] = a[I][k];
+}
+template <int I, typename T, int RR, int CC>
+__device__ __forceinline__ void row_set(T (&a)[RR][CC], const T (&v)[CC]) {
+#pragma unroll
+    for (int k = 0; k < CC; ++k) a[I][k] = v[k];
+}
+// Uniform-index dispatch: one switch case per register index.
+#define BH_DISPATCH16(idx, CALL)                                         \
+    switch (idx) {                                                        \
+        case 0: CALL(0); break;   case 1: CALL(1); break;                \
+        case 2: CALL(2); break;   case 3: CALL(3); break;                \
+        case 4: CALL(4); break;   case 5: CALL(5); break;                \
+        case 6: CALL(6); break;   case 7: CALL(7); break;                \
+        case 8: CALL(8); break;   case 9: CALL(9); break;                \
+        case 10: CALL(10); break; case 11: CALL(11); break;              \
+        case 12: CALL(12); break; case 13: CALL(13); break;              \
+        case 14: CALL(14); break; case 15: CALL(15); break;              \
+        default: break;                                                   \
+    }
+
+// Argmax over the RT consecutive lanes of a group (RT <= 16, groups aligned in
+// a DPP row): larger |value| wins, equal values keep the smaller position.
+// After the scan the group's last lane holds the winner.
+template <int RT, typename T>
+__device__ __forceinline__ void group_argmax(T &best, int &bpos, int &bslot, T &bval) {
+    const int g = (int)__lane_id() % RT;
+#define BH_ARGMAX_STEP(K, CTRL)                                                             \
+    if (RT > K) {                                                                           \
+        const T ob = dpp_mov<CTRL>(best, best), ov = dpp_mov<CTRL>(bval, bval);             \
+        const int op = __builtin_amdgcn_update_dpp(bpos, bpos, CTRL, 0xF, 0xF, false);     \
+        const int os = __builtin_amdgcn_update_dpp(bslot, bslot, CTRL, 0xF, 0xF, false);   \
+        if (g >= K && (ob > best || (ob == best && op < bpos))) {                           \
+            best = ob; bpos = op; bslot = os; bval = ov;                                    \
+        }                                                                                   \
+    }
+    BH_ARGMAX_STEP(1, 0x111)
+    BH_ARGMAX_STEP(2, 0x112)
+    BH_ARGMAX_STEP(4, 0x114)
+    BH_ARGMAX_STEP(8, 0x118)
+#undef BH_ARGMAX_STEP
+}
+
 // D = steps per I/O chunk: long enough to hide a global load behind a chunk
 // of steps, short enough that the staged rows fit in registers.
 template <typename T, int RT, int CT, int RR, int CC, int D>
@@ -160,28 +218,24 @@ __device__ void reg_gbtf2(const RegLuArgs<T> &s) {
         }
         const int buf = (int)(j & 1);
         const int jd = (int)(j % D);
-        const i64 km = imin(kl, m - 1 - j);
+        const int j32 = (int)j;
+        const int km = (int)imin(kl, m - 1 - j);
         const int ctj = jmodW % CT, kkj = jmodW / CT;
         // ------------------------------------------------------------ phase A
         // owners of column j: pivot search, multipliers, recycle the slot
         if (ct == ctj) {
             const int kk = __builtin_amdgcn_readfirstlane(kkj);
             T colv[RR];
-#pragma unroll
-            for (int i = 0; i < RR; ++i) {
-                T v = a[i][0];
-#pragma unroll
-                for (int k = 1; k < CC; ++k)
-                    if (kk == k) v = a[i][k];
-                colv[i] = v;
-            }
+#define BH_COLGET(K) if (K < CC) col_get<(K < CC ? K : 0)>(a, colv)
+            BH_DISPATCH16(kk, BH_COLGET)
+#undef BH_COLGET
             T best = T(-1);
             int bpos = 0x7fffffff, bslot = -1;
             T bval = T(0);
 #pragma unroll
             for (int i = 0; i < RR; ++i) {
                 const int p = pos[i];
-                if (p >= j && p <= j + km) {
+                if (p >= j32 && p <= j32 + km) {
                     const T cv = colv[i];
                     T v = dabs(cv);
                     if (!(v == v)) v = T(-0.5);  // NaN: a candidate that loses to any number
@@ -193,18 +247,13 @@ __device__ void reg_gbtf2(const RegLuArgs<T> &s) {
                     }
                 }
             }
-#pragma unroll
-            for (int off = RT / 2; off > 0; off >>= 1) {
-                const T ob = __shfl_xor(best, off, RT);
-                const int op = __shfl_xor(bpos, off, RT);
-                const int os = __shfl_xor(bslot, off, RT);
-                const T ov = __shfl_xor(bval, off, RT);
-                if (ob > best || (ob == best && op < bpos)) {
-                    best = ob;
-                    bpos = op;
-                    bslot = os;
-                    bval = ov;
-                }
+            group_argmax<RT>(best, bpos, bslot, bval);
+            {
+                // the group's result sits in its last lane; broadcast it
+                const int last = __builtin_amdgcn_readfirstlane(((ctj * RT) & 63) + RT - 1);
+                bpos = __builtin_amdgcn_readlane(bpos, last);
+                bslot = __builtin_amdgcn_readlane(bslot, last);
+                bval = lane_read(bval, last);
             }
             const int p = bpos, ps = bslot;
             const T pv = bval;
@@ -215,12 +264,12 @@ __device__ void reg_gbtf2(const RegLuArgs<T> &s) {
             for (int i = 0; i < RR; ++i) {
                 const int rs = rt + i * RT;
                 int np = pos[i];
-                if (nz && rs != ps && np == j) np = p;
-                if (rs != ps && np > j && np <= j + km) {
+                if (nz && rs != ps && np == j32) np = p;
+                if (rs != ps && np > j32 && np <= j32 + km) {
                     const T cv = colv[i];
                     const T l = nz ? cv * r : cv;
                     lbuf[buf][rs] = l;
-                    outL[jd * (kl + 1) + (np - j)] = l;
+                    outL[jd * (kl + 1) + (np - j32)] = l;
                 }
             }
             if (rt == 0) {
@@ -235,12 +284,10 @@ __device__ void reg_gbtf2(const RegLuArgs<T> &s) {
             const bool newcol = (j + kv < n);
             const T ev = rows[(int)((j + kl) % RB) * H + W];
 #pragma unroll
-            for (int i = 0; i < RR; ++i) {
-                const T v = (newcol && pos[i] == j + kl) ? ev : T(0);
-#pragma unroll
-                for (int k = 0; k < CC; ++k)
-                    if (kk == k) a[i][k] = v;
-            }
+            for (int i = 0; i < RR; ++i) colv[i] = (newcol && pos[i] == j32 + (int)kl) ? ev : T(0);
+#define BH_COLSET(K) if (K < CC) col_set<(K < CC ? K : 0)>(a, colv)
+            BH_DISPATCH16(kk, BH_COLSET)
+#undef BH_COLSET
         }
         lds_barrier();  // B1
         // ------------------------------------------------------------ phase B
@@ -254,7 +301,7 @@ __device__ void reg_gbtf2(const RegLuArgs<T> &s) {
 #pragma unroll
                 for (int i = 0; i < RR; ++i) {
                     const int rs = rt + i * RT;
-                    if (rs != ps && pos[i] == j) pos[i] = p;
+                    if (rs != ps && pos[i] == j32) pos[i] = p;
                 }
             }
         } else if (info == 0) {
@@ -271,12 +318,13 @@ __device__ void reg_gbtf2(const RegLuArgs<T> &s) {
         }
         if (rt == ps % RT) {
             const int i0 = __builtin_amdgcn_readfirstlane(ps / RT);
+            T urow[CC];
+#define BH_ROWGET(I) if (I < RR) row_get<(I < RR ? I : 0)>(a, urow)
+            BH_DISPATCH16(i0, BH_ROWGET)
+#undef BH_ROWGET
 #pragma unroll
             for (int k = 0; k < CC; ++k) {
-                T u = a[0][k];
-#pragma unroll
-                for (int i = 1; i < RR; ++i)
-                    if (i0 == i) u = a[i][k];
+                const T u = urow[k];
                 const i64 c = colk[k];
                 if (c < n) {
                     outU[jd * H + (int)(c - j)] = u;
@@ -298,7 +346,7 @@ __device__ void reg_gbtf2(const RegLuArgs<T> &s) {
             for (int i = 0; i < RR; ++i) {
                 const int pi = pos[i];
                 const int rs = rt + i * RT;
-                if (rs != ps && pi > j && pi <= j + km) {
+                if (rs != ps && pi > j32 && pi <= j32 + km) {
                     const T l = lbuf[buf][rs];
 #pragma unroll
                     for (int k = 0; k < CC; ++k)
@@ -311,17 +359,18 @@ __device__ void reg_gbtf2(const RegLuArgs<T> &s) {
             const int i0 = __builtin_amdgcn_readfirstlane(ps / RT);
             const i64 e = j + kl + 1;
             const T *rbe = rows + (int)(e % RB) * H;
+            T nv[CC];
 #pragma unroll
-            for (int i = 0; i < RR; ++i) {
-                if (i0 == i) {
-                    pos[i] = (e < m) ? (int)e : -1;
-#pragma unroll
-                    for (int k = 0; k < CC; ++k) {
-                        const i64 c = colk[k];
-                        a[i][k] = (e < m && c < n) ? rbe[c - j - 1] : T(0);
-                    }
-                }
+            for (int k = 0; k < CC; ++k) {
+                const i64 c = colk[k];
+                nv[k] = (e < m && c < n) ? rbe[c - j - 1] : T(0);
             }
+#pragma unroll
+            for (int i = 0; i < RR; ++i)
+                if (i0 == i) pos[i] = (e < m) ? (int)e : -1;
+#define BH_ROWSET(I) if (I < RR) row_set<(I < RR ? I : 0)>(a, nv)
+            BH_DISPATCH16(i0, BH_ROWSET)
+#undef BH_ROWSET
         }
         jmodW = jm1;
     }

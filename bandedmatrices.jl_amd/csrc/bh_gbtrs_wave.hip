@@ -21,32 +21,6 @@ namespace bh {
 
 constexpr int PF = 4;  // prefetch depth (steps)
 
-__device__ __forceinline__ double wave_shift_down(double v, double carry_in, int lane) {
-    // lane i <- lane i+1; lane 63 <- carry_in
-    double s = __shfl_down(v, 1, 64);
-    return lane == 63 ? carry_in : s;
-}
-__device__ __forceinline__ float wave_shift_down(float v, float carry_in, int lane) {
-    float s = __shfl_down(v, 1, 64);
-    return lane == 63 ? carry_in : s;
-}
-template <typename T>
-__device__ __forceinline__ T wave_shift_up(T v, T carry_in, int lane) {
-    // lane i <- lane i-1; lane 0 <- carry_in
-    T s = __shfl_up(v, 1, 64);
-    return lane == 0 ? carry_in : s;
-}
-template <typename T>
-__device__ __forceinline__ T lane_get(T v, int l) {
-    return __shfl(v, l, 64);
-}
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
-
 // Dynamic but wave-uniform register index (scalar branches).
 template <int N, typename T>
 __device__ __forceinline__ T reg_at(const T (&w)[N], int k) {
@@ -96,15 +70,15 @@ gbtrs_n_wave(i64 n, i64 kl, i64 ku, const T *__restrict__ ab, i64 ld, const i64 
                 const i64 j = j0 + s;
                 if (j < n - 1) {
                     const int d = (int)(pr[s] - j);
-                    T bj = lane_get(w[0], 0);
+                    T bj = lane_first(w[0]);
                     if (d != 0) {
-                        const int kd_ = __builtin_amdgcn_readfirstlane(d / 64), ld_ = d % 64;
-                        const T bp = lane_get(reg_at(w, kd_), ld_);
+                        const int kd_ = __builtin_amdgcn_readfirstlane(d / 64);
+                        const int ld_ = __builtin_amdgcn_readfirstlane(d % 64);
+                        const T bp = lane_read(reg_at(w, kd_), ld_);
+                        w[0] = lane_write(w[0], bp, 0);
 #pragma unroll
-                        for (int k = 0; k < NF; ++k) {
-                            if (k == 0 && lane == 0) w[0] = bp;
-                            if (k == kd_ && lane == ld_) w[k] = bj;
-                        }
+                        for (int k = 0; k < NF; ++k)
+                            if (k == kd_) w[k] = lane_write(w[k], bj, ld_);
                         bj = bp;
                     }
                     const T nb = -bj;
@@ -115,14 +89,14 @@ gbtrs_n_wave(i64 n, i64 kl, i64 ku, const T *__restrict__ ab, i64 ld, const i64 
                         if (q >= 1 && q <= lm) w[k] = dfma(lr[s][k], nb, w[k]);
                     }
                     // retire position 0 (final b_j)
-                    if (lane == (int)(j & 63)) outv = bj;
+                    outv = lane_write(outv, bj, (int)(j & 63));
                     if ((j & 63) == 63) b[j - 63 + lane] = outv;
                     // shift the window down by one position; a raw b enters at the top
                     const T vin = vr[s];
 #pragma unroll
                     for (int k = 0; k < NF; ++k) {
-                        const T carry = (k + 1 < NF) ? lane_get(w[k + 1], 0) : vin;
-                        w[k] = wave_shift_down(w[k], carry, lane);
+                        const T carry = (k + 1 < NF) ? lane_first(w[k + 1]) : vin;
+                        w[k] = wave_down1(w[k], carry);
                     }
                     fetch(s, j + PF);
                 }
@@ -170,7 +144,7 @@ gbtrs_n_wave(i64 n, i64 kl, i64 ku, const T *__restrict__ ab, i64 ld, const i64 
             for (int s = 0; s < PF; ++s) {
                 const i64 j = j0 - s;
                 if (j >= 0) {
-                    const T xj = lane_get(w[0], 0) / dr[s];
+                    const T xj = lane_first(w[0]) / dr[s];
                     const T nx = -xj;
                     const i64 qm = imin(kd, j);
 #pragma unroll
@@ -179,13 +153,13 @@ gbtrs_n_wave(i64 n, i64 kl, i64 ku, const T *__restrict__ ab, i64 ld, const i64 
                         if (q >= 1 && q <= qm) w[k] = dfma(ur[s][k], nx, w[k]);
                     }
                     const i64 r = top - j;  // retire count
-                    if (lane == (int)(r & 63)) outv = xj;
+                    outv = lane_write(outv, xj, (int)(r & 63));
                     if ((r & 63) == 63) b[j + 63 - lane] = outv;
                     const T vin = vr[s];
 #pragma unroll
                     for (int k = 0; k < NB; ++k) {
-                        const T carry = (k + 1 < NB) ? lane_get(w[k + 1], 0) : vin;
-                        w[k] = wave_shift_down(w[k], carry, lane);
+                        const T carry = (k + 1 < NB) ? lane_first(w[k + 1]) : vin;
+                        w[k] = wave_down1(w[k], carry);
                     }
                     fetch(s, j - PF);
                 }
@@ -231,12 +205,12 @@ gbtrs_t_wave(i64 n, i64 kl, i64 ku, const T *__restrict__ ab, i64 ld, const i64 
                     T part = T(0);
 #pragma unroll
                     for (int k = 0; k < NB; ++k) part = dfma(ur[s][k], w[k], part);
-                    const T xj = (br[s] - wave_sum(part)) / dr[s];
+                    const T xj = (br[s] - wave_sum_dpp(part)) / dr[s];
                     if (lane == 0) b[j] = xj;
 #pragma unroll
                     for (int k = NB - 1; k >= 0; --k) {
-                        const T carry = (k > 0) ? lane_get(w[k - 1], 63) : xj;
-                        w[k] = wave_shift_up(w[k], carry, lane);
+                        const T carry = (k > 0) ? lane_read(w[k - 1], 63) : xj;
+                        w[k] = wave_up1(w[k], carry);
                     }
                     fetch(s, j + PF);
                 }
@@ -273,30 +247,31 @@ gbtrs_t_wave(i64 n, i64 kl, i64 ku, const T *__restrict__ ab, i64 ld, const i64 
                 const i64 j = j0 - s;
                 if (j >= 0) {
                     // shift up: b_j enters at position 0, the top position retires
-                    const T top = lane_get(w[NF - 1], 63);
+                    const T top = lane_read(w[NF - 1], 63);
                     const i64 tq = j + 1 + 64 * NF - 1;  // index leaving the window
                     if (lane == 0 && tq < n) b[tq] = top;
                     const T vin = vr[s];
 #pragma unroll
                     for (int k = NF - 1; k >= 0; --k) {
-                        const T carry = (k > 0) ? lane_get(w[k - 1], 63) : vin;
-                        w[k] = wave_shift_up(w[k], carry, lane);
+                        const T carry = (k > 0) ? lane_read(w[k - 1], 63) : vin;
+                        w[k] = wave_up1(w[k], carry);
                     }
                     T part = T(0);
 #pragma unroll
                     for (int k = 0; k < NF; ++k) part = dfma(lr[s][k], w[k], part);
-                    const T sum = wave_sum(part);
-                    T bj = lane_get(w[0], 0) - sum;
+                    const T sum = wave_sum_dpp(part);
+                    T bj = lane_first(w[0]) - sum;
                     const int d = (int)(pr[s] - j);
                     if (d != 0) {
-                        const int kd_ = __builtin_amdgcn_readfirstlane(d / 64), ld_ = d % 64;
-                        const T bp = lane_get(reg_at(w, kd_), ld_);
+                        const int kd_ = __builtin_amdgcn_readfirstlane(d / 64);
+                        const int ld_ = __builtin_amdgcn_readfirstlane(d % 64);
+                        const T bp = lane_read(reg_at(w, kd_), ld_);
 #pragma unroll
                         for (int k = 0; k < NF; ++k)
-                            if (k == kd_ && lane == ld_) w[k] = bj;
+                            if (k == kd_) w[k] = lane_write(w[k], bj, ld_);
                         bj = bp;
                     }
-                    if (lane == 0) w[0] = bj;
+                    w[0] = lane_write(w[0], bj, 0);
                     fetch(s, j - PF);
                 }
             }
