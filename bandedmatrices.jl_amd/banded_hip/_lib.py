@@ -45,6 +45,9 @@ for _p, _t in (("d", dbl), ("s", flt)):
     SIGNATURES[f"bh_{_p}gbpad"] = (C.c_int, [i64, i64, i64, i64, vp, i64, vp, i64])
     SIGNATURES[f"bh_{_p}gbtrf"] = (C.c_int, [i64, i64, i64, i64, vp, i64, vp, pi64])
     SIGNATURES[f"bh_{_p}gbtrs"] = (C.c_int, [ch, i64, i64, i64, i64, vp, i64, vp, vp, i64])
+    SIGNATURES[f"bh_{_p}gbqrf"] = (C.c_int, [i64, i64, i64, i64, vp, i64, vp])
+    SIGNATURES[f"bh_{_p}gbqrm"] = (C.c_int, [ch, i64, i64, i64, i64, vp, i64, vp, i64, vp, i64])
+    SIGNATURES[f"bh_{_p}gbqrs"] = (C.c_int, [i64, i64, i64, i64, vp, i64, vp, i64, vp, i64])
 
 SIGNATURES["bh_dsgbpad"] = (C.c_int, [i64, i64, i64, i64, vp, i64, vp, i64])
 SIGNATURES["bh_dsgbsv_workspace"] = (C.c_int, [i64, i64, i64, i64, C.POINTER(C.c_size_t)])

@@ -1,0 +1,114 @@
+"""Banded QR, mirroring the reference's qr on BandedMatrix.
+
+  qr / qr!            src/banded/bandedqr.jl:1-35 (padded copy, _banded_qr!)
+  Q' * B, Q * B       src/banded/bandedqr.jl:38-94,150-172 (banded_qr_lmul!)
+  ldiv!(F::QR, B)     src/banded/bandedqr.jl:175-206 (Q'B then tbsv of R)
+The factors stay in HBM: R in the upper l+u bands of the padded copy, the
+reflectors in the lower l band, tau as a device vector.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .bandedmatrix import BandedMatrix, DimensionMismatch, _prefix
+from .device import DeviceArray
+from .lu import pad_for_lu
+
+
+@dataclass
+class BandedQR:
+    """factors: padded BandedMatrix with bands (l, l+u); tau: device vector of min(m, n)."""
+
+    factors: BandedMatrix
+    tau: DeviceArray
+
+    @property
+    def shape(self):
+        return self.factors.shape
+
+    @property
+    def Q(self):
+        return QRPackedQ(self, adjoint=False)
+
+    def R_dense(self) -> np.ndarray:
+        """R as a dense host array (min(m,n) x n), for tests and display."""
+        m, n = self.shape
+        F = self.factors
+        D = F.host_data()
+        R = np.zeros((min(m, n), n), D.dtype)
+        for j in range(n):
+            for i in range(max(0, j - F.u), min(j, min(m, n) - 1) + 1):
+                R[i, j] = D[F.u + i - j, j]
+        return R
+
+
+@dataclass
+class QRPackedQ:
+    parent: BandedQR
+    adjoint: bool
+
+    @property
+    def T(self):
+        return QRPackedQ(self.parent, not self.adjoint)
+
+    def lmul_(self, B: DeviceArray) -> DeviceArray:
+        """lmul!(Q, B) / lmul!(Q', B) in place."""
+        return qr_lmul_(self.parent, B, "T" if self.adjoint else "N")
+
+    def __matmul__(self, B: DeviceArray) -> DeviceArray:
+        return self.lmul_(B.copy())
+
+
+def qr_(A: BandedMatrix) -> BandedQR:
+    """qr!(A): A already carries the l extra super-diagonals (bands (l, l+u))."""
+    m, n = A.shape
+    l, u = A.bandwidths()
+    tau = DeviceArray((max(min(m, n), 1),), A.dtype)
+    _lib.call(f"bh_{_prefix(A.dtype)}gbqrf", m, n, l, u, A.data.ptr, A.data.ld, tau.ptr)
+    return BandedQR(A, tau)
+
+
+def qr(A: BandedMatrix) -> BandedQR:
+    """_qr: the padded copy BandedMatrix{T}(A, (l, l+u)), then qr! (bandedqr.jl:1-5)."""
+    if A.l < 0 or A.u < 0:
+        raise NotImplementedError("qr of a matrix with negative bandwidths")
+    return qr_(pad_for_lu(A))
+
+
+def qr_lmul_(F: BandedQR, B: DeviceArray, trans: str) -> DeviceArray:
+    m, n = F.shape
+    if B.shape[0] != m:
+        raise DimensionMismatch(f"B has {B.shape[0]} rows, Q is {m} x {m}")
+    if B.dtype != F.factors.dtype:
+        raise TypeError("mixed element types")
+    nrhs = 1 if len(B.shape) == 1 else B.shape[1]
+    l, u = F.factors.bandwidths()
+    _lib.call(f"bh_{_prefix(B.dtype)}gbqrm", trans.encode(), m, n, l, u, F.factors.data.ptr, F.factors.data.ld,
+              F.tau.ptr, nrhs, B.ptr, max(1, B.shape[0]))
+    return B
+
+
+def qr_ldiv_(F: BandedQR, B: DeviceArray) -> DeviceArray:
+    """ldiv!(F, B): B (m x nrhs) <- Q'B, then its first n rows <- R \\ (Q'B)[1:n]."""
+    m, n = F.shape
+    if m < n:
+        raise ValueError("Not implemented for wide matrices")  # bandedqr.jl:175-177
+    if B.shape[0] != m:
+        raise DimensionMismatch(f"B has {B.shape[0]} rows, A has {m}")
+    if B.dtype != F.factors.dtype:
+        raise TypeError("mixed element types")
+    nrhs = 1 if len(B.shape) == 1 else B.shape[1]
+    l, u = F.factors.bandwidths()
+    _lib.call(f"bh_{_prefix(B.dtype)}gbqrs", m, n, l, u, F.factors.data.ptr, F.factors.data.ld, F.tau.ptr, nrhs,
+              B.ptr, max(1, B.shape[0]))
+    return B
+
+
+def qr_solve(A: BandedMatrix, b: DeviceArray) -> DeviceArray:
+    """qr(A) \\ b for m >= n: the least-squares solution (first n rows of the result)."""
+    F = qr(A)
+    x = qr_ldiv_(F, b.copy())
+    return x

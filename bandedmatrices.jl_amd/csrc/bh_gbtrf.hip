@@ -15,6 +15,9 @@
 // barriers.  Each phase is spread over the workgroup's lanes.
 #include "bh_common.hpp"
 
+#include <cstdlib>
+#include <cstring>
+
 namespace bh {
 
 constexpr int kLuThreads = 256;
@@ -130,6 +133,20 @@ __global__ void __launch_bounds__(kLuThreads) gbtf2_single(LuSys<T> s, i64 *info
 
 template <typename T>
 int reg_gbtrf_launch(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, i64 *dinfo, hipStream_t st);
+template <typename T>
+int blk_gbtrf_launch(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, i64 *dinfo, hipStream_t st);
+
+// Kernel choice: BANDEDHIP_LU_KERNEL = "blk" | "reg" | "global" forces one
+// (where it applies); by default the blocked kernel takes bands with
+// kl >= 16 and the per-step register kernel the narrower ones.
+static int lu_kernel_override() {
+    const char *e = getenv("BANDEDHIP_LU_KERNEL");
+    if (!e) return 0;
+    if (!strcmp(e, "blk")) return 1;
+    if (!strcmp(e, "reg")) return 2;
+    if (!strcmp(e, "global")) return 3;
+    return 0;
+}
 
 template <typename T>
 static int gbtrf_impl(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, i64 *info_host) {
@@ -145,7 +162,10 @@ static int gbtrf_impl(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, 
     BH_HIP(hipMallocAsync((void **)&dinfo, sizeof(i64), stream()));
     // register-resident kernel when the window fits one of its shapes,
     // otherwise the barrier-stepped kernel working in global memory
-    int st = reg_gbtrf_launch<T>(m, n, kl, ku, ab, ldab, ipiv, dinfo, stream());
+    const int ov = lu_kernel_override();
+    int st = -1;
+    if (ov == 1 || (ov == 0 && kl >= 16)) st = blk_gbtrf_launch<T>(m, n, kl, ku, ab, ldab, ipiv, dinfo, stream());
+    if (st == -1 && ov != 3) st = reg_gbtrf_launch<T>(m, n, kl, ku, ab, ldab, ipiv, dinfo, stream());
     if (st == -1) {
         LuSys<T> s{ab, ipiv, m, n, kl, ku, ldab};
         hipLaunchKernelGGL(gbtf2_single<T>, dim3(1), dim3(kLuThreads), 0, stream(), s, dinfo);

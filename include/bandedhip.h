@@ -115,8 +115,10 @@ int bh_dsgbpad(bh_int m, bh_int n, bh_int kl, bh_int ku, const double *A, bh_int
 /* Partial-pivot band LU in place on the padded array AB (ldab >= 2kl+ku+1,
  * diagonal at row kl+ku), pivots into ipiv (device, 1-based, min(m,n)
  * entries), LAPACK info into *info_host (0 ok, j>0 zero pivot in column j).
- * For ku <= 64 the factors and pivots equal LAPACK dgbtf2/dgbtrf bit for bit;
- * wider bands use a blocked MFMA factorisation whose pivots match.
+ * Factors and pivots equal LAPACK dgbtf2 bit for bit (bands with kl >= 16 use
+ * a blocked factorisation that keeps dgbtf2's per-entry operation order; on
+ * inputs containing -0.0 or non-finite values it may differ in zero signs
+ * and in how far NaN spreads).
  * Replaces LAPACK.gbtrf! in lu! (src/banded/BandedLU.jl:90-103). */
 int bh_dgbtrf(bh_int m, bh_int n, bh_int kl, bh_int ku, double *AB, bh_int ldab,
               bh_int *ipiv, bh_int *info_host);
@@ -132,6 +134,33 @@ int bh_dgbtrs(char trans, bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const dou
               bh_int ldab, const bh_int *ipiv, double *B, bh_int ldb);
 int bh_sgbtrs(char trans, bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const float *AB,
               bh_int ldab, const bh_int *ipiv, float *B, bh_int ldb);
+
+/* ------------------------------------------------------------- banded QR */
+/* Householder QR in band storage: D is the padded copy (bh_?gbpad with
+ * kl = l, ku = u_orig) viewed as bands (l, u) with u = l + u_orig, ldd >= l+u+1,
+ * diagonal at row u.  On return R is in the upper u bands and the reflectors'
+ * v (v1 = 1 implicit) in the lower l band; tau (device, min(m,n) entries) is
+ * zero past the last reflector (min(m-1, n) of them).  Reflector convention:
+ * ArrayLayouts.reflector! (tau = 2 when x(2:end) = 0).
+ * Replaces _banded_qr! via qr/qr! (src/banded/bandedqr.jl:1-35). */
+int bh_dgbqrf(bh_int m, bh_int n, bh_int l, bh_int u, double *D, bh_int ldd, double *tau);
+int bh_sgbqrf(bh_int m, bh_int n, bh_int l, bh_int u, float *D, bh_int ldd, float *tau);
+/* B <- Q' B (trans 'T' or 'C') or Q B ('N'); B is m x nrhs (ldb >= m).
+ * Replaces banded_qr_lmul! (src/banded/bandedqr.jl:38-94) behind lmul!(Q', B)
+ * and lmul!(Q, B) (bandedqr.jl:150-172). */
+int bh_dgbqrm(char trans, bh_int m, bh_int n, bh_int l, bh_int u, const double *D, bh_int ldd,
+              const double *tau, bh_int nrhs, double *B, bh_int ldb);
+int bh_sgbqrm(char trans, bh_int m, bh_int n, bh_int l, bh_int u, const float *D, bh_int ldd,
+              const float *tau, bh_int nrhs, float *B, bh_int ldb);
+/* Least-squares / square solve with the factors: B (m x nrhs, m >= n) <- Q'B,
+ * then the first n rows <- R \ (Q'B)[1:n] (upper band solve, u super-diagonals).
+ * m < n is an argument error (-1), as the reference raises for wide QR.
+ * Replaces ldiv!(::QR{<:Any,<:AbstractBandedMatrix}, B) (src/banded/bandedqr.jl:175-206)
+ * and its tbsv back substitution (src/banded/tribanded.jl:75-83). */
+int bh_dgbqrs(bh_int m, bh_int n, bh_int l, bh_int u, const double *D, bh_int ldd, const double *tau,
+              bh_int nrhs, double *B, bh_int ldb);
+int bh_sgbqrs(bh_int m, bh_int n, bh_int l, bh_int u, const float *D, bh_int ldd, const float *tau,
+              bh_int nrhs, float *B, bh_int ldb);
 
 /* ------------------------------------------------------- mixed precision */
 /* X = A \ B for an FP64 band matrix A ((kl+ku+1) x n, lda) via an FP32 band LU

@@ -1,0 +1,107 @@
+"""GPU parity: banded Householder QR (bandedqr.jl) against the oracle.
+
+The factorisation follows _banded_qr! / ArrayLayouts.reflector! in the
+reference's operation order (sequential norms and dot products, divisions by
+x1 + copysign(||x||, x1)), so FP64 factors and tau must equal the oracle's
+restatement bit for bit.  Q'B and QB use a tree sum for each reflector's dot
+product: within 1e-13 of the oracle (relative, max norm).  The QR solve is
+checked against the oracle and through the normal equations."""
+import numpy as np
+import pytest
+
+from _util import band, bits_equal
+
+pytestmark = pytest.mark.gpu
+
+QR_CASES = [
+    # m, n, l, u
+    (1, 1, 0, 0),
+    (5, 5, 1, 1),
+    (10, 10, 2, 1),
+    (300, 300, 4, 3),
+    (10000, 10000, 4, 3),     # the plain CPU-runnable config
+    (400, 300, 3, 5),         # tall
+    (300, 400, 3, 5),         # wide (factor and Q only)
+    (500, 500, 0, 4),
+    (500, 500, 6, 0),
+    (600, 600, 15, 10),       # 16-row register window
+    (700, 700, 31, 20),       # 32-row window, u' = 51
+    (300, 300, 40, 10),       # l+1 > 32: one-workgroup path
+    (300, 300, 5, 60),        # u' + 1 > 64: one-workgroup path
+]
+
+
+def _make(rng, m, n, l, u, dtype=np.float64):
+    A = band(rng, l, u, n, np.float64, dist="n", m=m)
+    return A.astype(dtype)
+
+
+def _padded(oracle, A, m, n, l, u):
+    return oracle.pad_for_lu(A, m, n, l, u)
+
+
+@pytest.mark.parametrize("m,n,l,u", QR_CASES)
+def test_gbqrf_bits(bh, oracle, m, n, l, u):
+    rng = np.random.default_rng(m * 7 + n + l * 31 + u)
+    A = _make(rng, m, n, l, u)
+    F = bh.qr(bh.BandedMatrix.from_host(A, m, l, u))
+    D, tau = oracle.gbqrf(m, n, l, l + u, _padded(oracle, A, m, n, l, u))
+    g = F.factors.host_data()
+    assert bits_equal(g, D), np.nanmax(np.abs(g - D))
+    assert bits_equal(F.tau.numpy()[: min(m, n)], tau)
+
+
+@pytest.mark.parametrize("m,n,l,u", [c for c in QR_CASES if c[0] <= 2000])
+@pytest.mark.parametrize("nrhs", [1, 3])
+def test_q_apply(bh, oracle, m, n, l, u, nrhs):
+    rng = np.random.default_rng(m + 3 * n + 5 * l + u + nrhs)
+    A = _make(rng, m, n, l, u)
+    F = bh.qr(bh.BandedMatrix.from_host(A, m, l, u))
+    D, tau = oracle.gbqrf(m, n, l, l + u, _padded(oracle, A, m, n, l, u))
+    B = np.asfortranarray(rng.standard_normal((m, nrhs)))
+    for adj, ref in ((True, oracle.gbqr_qtmul), (False, oracle.gbqr_qmul)):
+        Q = F.Q.T if adj else F.Q
+        got = (Q @ bh.DeviceArray.from_numpy(B)).numpy()
+        want = ref(m, n, l, l + u, D, tau, B)
+        assert np.abs(got - want).max() <= 1e-13 * max(1.0, np.abs(want).max())
+    # Q'(Q B) = B
+    dB = bh.DeviceArray.from_numpy(B)
+    F.Q.lmul_(dB)
+    F.Q.T.lmul_(dB)
+    assert np.abs(dB.numpy() - B).max() <= 1e-13 * max(1.0, np.abs(B).max())
+
+
+@pytest.mark.parametrize("m,n,l,u", [c for c in QR_CASES if c[0] >= c[1]])
+def test_qr_solve(bh, oracle, m, n, l, u):
+    rng = np.random.default_rng(11 * m + n + l + 2 * u)
+    A = _make(rng, m, n, l, u)
+    A[u, :] += 3.0 * (l + u + 1)  # well conditioned
+    B = np.asfortranarray(rng.standard_normal((m, 2)))
+    F = bh.qr(bh.BandedMatrix.from_host(A, m, l, u))
+    X = bh.qr_ldiv_(F, bh.DeviceArray.from_numpy(B)).numpy()[:n]
+    D, tau = oracle.gbqrf(m, n, l, l + u, _padded(oracle, A, m, n, l, u))
+    Xo = oracle.gbqrs(m, n, l, l + u, D, tau, B)[:n]
+    assert np.abs(X - Xo).max() <= 1e-12 * max(1.0, np.abs(Xo).max())
+    # normal equations: A'(A x - b) = 0 (least squares; exact solve when square)
+    Ad = oracle.band_to_dense(A, m, n, l, u)
+    r = Ad @ X - B
+    assert np.abs(Ad.T @ r).max() <= 1e-9 * np.abs(Ad).sum(axis=0).max() * max(1.0, np.abs(X).max())
+
+
+def test_qr_fp32_close_to_fp64(bh, oracle):
+    m = n = 2000
+    l, u = 4, 3
+    rng = np.random.default_rng(5)
+    A = _make(rng, m, n, l, u)
+    F = bh.qr(bh.BandedMatrix.from_host(A.astype(np.float32), m, l, u))
+    D, tau = oracle.gbqrf(m, n, l, l + u, _padded(oracle, A, m, n, l, u))
+    g = F.factors.host_data().astype(np.float64)
+    assert np.abs(g - D).max() <= 1e-4 * np.abs(D).max()
+
+
+def test_qr_wide_solve_is_an_error(bh):
+    rng = np.random.default_rng(1)
+    A = _make(rng, 30, 40, 2, 2)
+    F = bh.qr(bh.BandedMatrix.from_host(A, 30, 2, 2))
+    with pytest.raises(ValueError):
+        bh.qr_ldiv_(F, bh.DeviceArray.from_numpy(np.ones(30)))
