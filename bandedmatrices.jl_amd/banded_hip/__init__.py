@@ -11,6 +11,7 @@ from .bandedmatrix import (BandedMatrix, BandError, DimensionMismatch, _BandedMa
                            prodbandwidths)
 from .lu import BandedLU, SingularException, TransposeFact, ldiv_, lu, lu_, pad_for_lu, solve
 from .mixed import MixedResult, MixedSolver, solve_mixed
+from .batch import BatchedGBSV, BatchGraph, c4_batch
 from .qr import BandedQR, QRPackedQ, qr, qr_, qr_ldiv_, qr_lmul_, qr_solve
 
 __all__ = [
@@ -18,4 +19,5 @@ __all__ = [
     "DimensionMismatch", "_BandedMatrix", "brand", "gbmm", "mul", "prodbandwidths", "BandedLU",
     "SingularException", "TransposeFact", "ldiv_", "lu", "lu_", "pad_for_lu", "solve", "MixedResult",
     "MixedSolver", "solve_mixed", "BandedQR", "QRPackedQ", "qr", "qr_", "qr_ldiv_", "qr_lmul_", "qr_solve",
+    "BatchedGBSV", "BatchGraph", "c4_batch",
 ]

@@ -49,6 +49,18 @@ for _p, _t in (("d", dbl), ("s", flt)):
     SIGNATURES[f"bh_{_p}gbqrm"] = (C.c_int, [ch, i64, i64, i64, i64, vp, i64, vp, i64, vp, i64])
     SIGNATURES[f"bh_{_p}gbqrs"] = (C.c_int, [i64, i64, i64, i64, vp, i64, vp, i64, vp, i64])
 
+SIGNATURES["bh_batch_plan_create"] = (C.c_int, [i64, vp, vp, vp, C.POINTER(vp)])
+SIGNATURES["bh_batch_plan_layout"] = (C.c_int, [vp, vp, vp, pi64, pi64])
+SIGNATURES["bh_batch_plan_schedule"] = (C.c_int, [vp, C.c_int, C.c_int, C.c_int])
+SIGNATURES["bh_batch_plan_get_schedule"] = (C.c_int, [vp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                                      C.POINTER(C.c_int)])
+SIGNATURES["bh_batch_plan_destroy"] = (C.c_int, [vp])
+for _p in ("d", "s"):
+    SIGNATURES[f"bh_{_p}gbsv_batched"] = (C.c_int, [vp, vp, vp, vp, vp])
+    SIGNATURES[f"bh_{_p}gbsv_batched_autotune"] = (C.c_int, [vp, vp, vp, C.POINTER(dbl)])
+    SIGNATURES[f"bh_{_p}gbsv_batched_graph"] = (C.c_int, [vp, vp, vp, vp, vp, C.POINTER(vp)])
+SIGNATURES["bh_batch_graph_launch"] = (C.c_int, [vp])
+SIGNATURES["bh_batch_graph_destroy"] = (C.c_int, [vp])
 SIGNATURES["bh_dsgbpad"] = (C.c_int, [i64, i64, i64, i64, vp, i64, vp, i64])
 SIGNATURES["bh_dsgbsv_workspace"] = (C.c_int, [i64, i64, i64, i64, C.POINTER(C.c_size_t)])
 SIGNATURES["bh_dsgbsv"] = (C.c_int, [i64, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp, C.c_size_t, vp, pi64, pi64])

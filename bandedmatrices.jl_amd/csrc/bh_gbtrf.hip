@@ -135,16 +135,20 @@ template <typename T>
 int reg_gbtrf_launch(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, i64 *dinfo, hipStream_t st);
 template <typename T>
 int blk_gbtrf_launch(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, i64 *dinfo, hipStream_t st);
+template <typename T>
+int small_gbtrf_launch(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, i64 *dinfo, hipStream_t st);
 
-// Kernel choice: BANDEDHIP_LU_KERNEL = "blk" | "reg" | "global" forces one
-// (where it applies); by default the blocked kernel takes bands with
-// kl >= 16 and the per-step register kernel the narrower ones.
+// Kernel choice: BANDEDHIP_LU_KERNEL = "small" | "blk" | "reg" | "global"
+// forces one (where it applies).  By default: the one-wave kernel for
+// windows of at most 64 columns (kl + ku <= 63, kl <= 31), the blocked
+// kernel for kl >= 16, the per-step register kernel otherwise.
 static int lu_kernel_override() {
     const char *e = getenv("BANDEDHIP_LU_KERNEL");
     if (!e) return 0;
     if (!strcmp(e, "blk")) return 1;
     if (!strcmp(e, "reg")) return 2;
     if (!strcmp(e, "global")) return 3;
+    if (!strcmp(e, "small")) return 4;
     return 0;
 }
 
@@ -164,7 +168,9 @@ static int gbtrf_impl(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, 
     // otherwise the barrier-stepped kernel working in global memory
     const int ov = lu_kernel_override();
     int st = -1;
-    if (ov == 1 || (ov == 0 && kl >= 16)) st = blk_gbtrf_launch<T>(m, n, kl, ku, ab, ldab, ipiv, dinfo, stream());
+    if (ov == 4 || ov == 0) st = small_gbtrf_launch<T>(m, n, kl, ku, ab, ldab, ipiv, dinfo, stream());
+    if (st == -1 && (ov == 1 || (ov == 0 && kl >= 16)))
+        st = blk_gbtrf_launch<T>(m, n, kl, ku, ab, ldab, ipiv, dinfo, stream());
     if (st == -1 && ov != 3) st = reg_gbtrf_launch<T>(m, n, kl, ku, ab, ldab, ipiv, dinfo, stream());
     if (st == -1) {
         LuSys<T> s{ab, ipiv, m, n, kl, ku, ldab};

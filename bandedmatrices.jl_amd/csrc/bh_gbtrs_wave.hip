@@ -17,6 +17,9 @@
 //   -- dot products are wave reductions (summation order differs from BLAS).
 #include "bh_common.hpp"
 
+#include <cstdlib>
+#include <cstring>
+
 namespace bh {
 
 constexpr int PF = 4;  // prefetch depth (steps)
@@ -286,8 +289,20 @@ gbtrs_t_wave(i64 n, i64 kl, i64 ku, const T *__restrict__ ab, i64 ld, const i64 
 }
 
 template <typename T>
+int ring_gbtrs_launch(i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 ld, const i64 *ipiv, T *B, i64 ldb,
+                      hipStream_t st);
+
+template <typename T>
 int wave_gbtrs_launch(bool trans_n, i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 ld, const i64 *ipiv, T *B,
                       i64 ldb, hipStream_t st) {
+    // 'N': the lane-ring kernel unless BANDEDHIP_TRS_KERNEL=shift asks for this file's
+    if (trans_n) {
+        const char *e = getenv("BANDEDHIP_TRS_KERNEL");
+        if (!(e && !strcmp(e, "shift"))) {
+            const int r = ring_gbtrs_launch<T>(n, kl, ku, nrhs, ab, ld, ipiv, B, ldb, st);
+            if (r != -1) return r;
+        }
+    }
     const i64 kd = kl + ku;
     const int nf = (int)cdiv(kl + 1, 64), nb = (int)cdiv(kd + 1, 64);
     if (nf > 3 || nb > 5) return -1;

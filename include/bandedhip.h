@@ -162,6 +162,48 @@ int bh_dgbqrs(bh_int m, bh_int n, bh_int l, bh_int u, const double *D, bh_int ld
 int bh_sgbqrs(bh_int m, bh_int n, bh_int l, bh_int u, const float *D, bh_int ldd, const float *tau,
               bh_int nrhs, float *B, bh_int ldb);
 
+/* ------------------------------------------------------------- batched */
+/* Many independent square systems A_s x_s = b_s (kl_s <= 8, kl_s + ku_s <= 63)
+ * factored and solved in one persistent launch: each of the W resident waves
+ * walks its share of the schedule (systems w, w+W, ...; by default ordered by
+ * decreasing work and dealt out in snake rounds), running the band LU with
+ * the forward substitution fused into the factorisation sweep, then the back
+ * substitution.  Per system the results equal LAPACK dgbtf2 +
+ * dgbtrs 'N' bit for bit.
+ * Packed layout (bh_batch_plan_layout): system s's padded AB (ldab = 2kl+ku+1,
+ * as bh_?gbpad writes it) at AB + ab_off[s], its b / ipiv (1-based) at
+ * B + vec_off[s] / ipiv + vec_off[s]; info[s] is LAPACK's info.
+ * No reference counterpart (every reference call factors one matrix,
+ * src/banded/BandedLU.jl:90-111); callers are loops of lu(A_s) \ b_s. */
+typedef struct bh_batch_plan bh_batch_plan;
+typedef struct bh_batch_graph bh_batch_graph;
+int bh_batch_plan_create(bh_int nsys, const bh_int *n, const bh_int *kl, const bh_int *ku,
+                         bh_batch_plan **plan);
+int bh_batch_plan_layout(const bh_batch_plan *plan, bh_int *ab_off, bh_int *vec_off,
+                         bh_int *ab_elems, bh_int *vec_elems);
+/* Schedule: order 0 = decreasing work (snake rounds), 1 = as given
+ * (round-robin), 2 = band classes interleaved; waves per workgroup (1..4);
+ * resident workgroups. */
+int bh_batch_plan_schedule(bh_batch_plan *plan, int order_kind, int waves_per_block, int blocks);
+int bh_batch_plan_get_schedule(const bh_batch_plan *plan, int *order_kind, int *waves_per_block,
+                               int *blocks);
+int bh_batch_plan_destroy(bh_batch_plan *plan);
+int bh_dgbsv_batched(bh_batch_plan *plan, double *AB, double *B, bh_int *ipiv, bh_int *info);
+int bh_sgbsv_batched(bh_batch_plan *plan, float *AB, float *B, bh_int *ipiv, bh_int *info);
+/* Time the candidate schedules on scratch copies of AB / B (the inputs are
+ * not modified) and keep the fastest in the plan; *best_ms its time. */
+int bh_dgbsv_batched_autotune(bh_batch_plan *plan, const double *AB, const double *B, double *best_ms);
+int bh_sgbsv_batched_autotune(bh_batch_plan *plan, const float *AB, const float *B, double *best_ms);
+/* Capture the batched solve (one kernel node, no memset node) into a graph
+ * bound to these buffers; replay with bh_batch_graph_launch on the library
+ * stream. */
+int bh_dgbsv_batched_graph(bh_batch_plan *plan, double *AB, double *B, bh_int *ipiv, bh_int *info,
+                           bh_batch_graph **graph);
+int bh_sgbsv_batched_graph(bh_batch_plan *plan, float *AB, float *B, bh_int *ipiv, bh_int *info,
+                           bh_batch_graph **graph);
+int bh_batch_graph_launch(bh_batch_graph *graph);
+int bh_batch_graph_destroy(bh_batch_graph *graph);
+
 /* ------------------------------------------------------- mixed precision */
 /* X = A \ B for an FP64 band matrix A ((kl+ku+1) x n, lda) via an FP32 band LU
  * and FP64 iterative refinement: residuals B - A X in FP64, corrections from
