@@ -11,6 +11,7 @@ from .bandedmatrix import (BandedMatrix, BandError, DimensionMismatch, _BandedMa
                            prodbandwidths)
 from .lu import BandedLU, SingularException, TransposeFact, ldiv_, lu, lu_, pad_for_lu, solve
 from .mixed import MixedResult, MixedSolver, solve_mixed
+from .adjoint import band_outer_, loss_and_grad, mul_pullback, solve_pullback
 from .batch import BatchedGBSV, BatchGraph, c4_batch
 from .qr import BandedQR, QRPackedQ, qr, qr_, qr_ldiv_, qr_lmul_, qr_solve
 
@@ -19,5 +20,5 @@ __all__ = [
     "DimensionMismatch", "_BandedMatrix", "brand", "gbmm", "mul", "prodbandwidths", "BandedLU",
     "SingularException", "TransposeFact", "ldiv_", "lu", "lu_", "pad_for_lu", "solve", "MixedResult",
     "MixedSolver", "solve_mixed", "BandedQR", "QRPackedQ", "qr", "qr_", "qr_ldiv_", "qr_lmul_", "qr_solve",
-    "BatchedGBSV", "BatchGraph", "c4_batch",
+    "BatchedGBSV", "BatchGraph", "c4_batch", "band_outer_", "loss_and_grad", "mul_pullback", "solve_pullback",
 ]

@@ -59,6 +59,9 @@ for _p in ("d", "s"):
     SIGNATURES[f"bh_{_p}gbsv_batched"] = (C.c_int, [vp, vp, vp, vp, vp])
     SIGNATURES[f"bh_{_p}gbsv_batched_autotune"] = (C.c_int, [vp, vp, vp, C.POINTER(dbl)])
     SIGNATURES[f"bh_{_p}gbsv_batched_graph"] = (C.c_int, [vp, vp, vp, vp, vp, C.POINTER(vp)])
+for _p, _t in (("d", dbl), ("s", flt)):
+    SIGNATURES[f"bh_{_p}gb_band_grad"] = (C.c_int, [i64, i64, i64, i64, vp, vp, vp, vp, vp, i64])
+    SIGNATURES[f"bh_{_p}gb_band_outer"] = (C.c_int, [i64, i64, i64, i64, _t, vp, vp, _t, vp, vp, _t, vp, i64])
 SIGNATURES["bh_batch_graph_launch"] = (C.c_int, [vp])
 SIGNATURES["bh_batch_graph_destroy"] = (C.c_int, [vp])
 SIGNATURES["bh_dsgbpad"] = (C.c_int, [i64, i64, i64, i64, vp, i64, vp, i64])

@@ -1,20 +1,26 @@
 """Benchmark of the banded hot path on one MI355X.
 
-Headline (value): mixed-precision A\\b solves/s for the precision workload
+Headline (value): mixed-precision A\\b solves/s on the precision workload
   n = 65536, l = u = 128, band entries iid N(0,1) with the diagonal shifted by
-  +2(l+u), 16 right-hand sides iid N(0,1); one step = one complete A\\b
-  (FP32 band LU + FP64 iterative refinement to LAPACK dsgesv's backward-error
-  test), inputs resident in HBM.  The plain FP64 GPU A\\b (band LU + solve) is
-  timed beside it, and both solutions are compared with the FP64 CPU solution
-  (forward error, max norm).
+  +2(l+u), 16 right-hand sides iid N(0,1); one step = one complete A\\b (FP32
+  band LU + FP64 iterative refinement to LAPACK dsgesv's backward-error test),
+  inputs resident in HBM.  The plain FP64 GPU A\\b (band LU + solve) is timed
+  beside it; both solutions are compared with the FP64 CPU solution (forward
+  error, max norm) and the FP64 pivots with the CPU's.
 Second half of the metric: FP64 gbmv (8,8) at n = 2^20 in HBM GB/s
-  (algorithmic bytes 152 n = A + x + y), buffers rotated so every launch
-  reads from HBM rather than the 256 MB last-level cache.
-cpu_baseline: the reference's CPU path for the same solve -- OpenBLAS
-  dgbtrf + dgbtrs (the LAPACK routines BandedLU.jl:98 and linalg.jl:28 call),
-  all host cores.
+  (algorithmic bytes 152 n = A + x + y), buffers rotated so that every launch
+  reads HBM rather than the 256 MB last-level cache.
+Beside them (extra fields): gbmv FP32 / 64 right-hand sides and A*B (8,8)x(4,4)
+  at n = 2^20; the batch of 4096 systems (one persistent launch, autotuned);
+  the gradient of dot(w, A\\b) + 0.5||A x0||^2 at n = 262144, l = u = 16; the
+  plain n = 10^4, (4,3) case.
+cpu_baseline: the reference's CPU path for the headline solve -- OpenBLAS
+  dgbtrf + dgbtrs (the LAPACK routines BandedLU.jl:98 and linalg.jl:28 call)
+  on all host cores; per-workload CPU timings sit in each extra field.
 
 Prints one JSON line.  Only one GPU is supported: --gpus N > 1 exits.
+`--traffic-probe NAME` runs only a few launches of one kernel and a 256 MiB
+device copy (the scale for rocprofv3 FETCH_SIZE / WRITE_SIZE counters).
 """
 from __future__ import annotations
 
@@ -33,6 +39,11 @@ for _p in (ROOT, os.path.join(ROOT, "bandedmatrices.jl_amd")):
 
 HBM_PEAK_GBPS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def make_precision_problem(n=65536, l=128, u=128, nrhs=16, shift=True, seed=0):
@@ -45,8 +56,12 @@ def make_precision_problem(n=65536, l=128, u=128, nrhs=16, shift=True, seed=0):
 
 
 def lu_flops(n, l, u):
-    """Multiply-add count x2 of an unpivoted band LU: scale l + update 2 l u per column."""
+    """Multiply-adds x2 of a band LU without row exchanges: scale l + update 2 l u per column."""
     return float(n) * (l + 2.0 * l * u)
+
+
+def solve_flops(n, l, u, nrhs):
+    return float(nrhs) * 2.0 * n * (2 * l + u)
 
 
 def cpu_reference_solve(A, B, l, u, threads):
@@ -65,30 +80,255 @@ def cpu_reference_solve(A, B, l, u, threads):
     return x, piv + 1, t1 - t0
 
 
-def bench_gbmv(bh, steps, warmup):
-    """FP64 gbmv (8,8) at n = 2^20; 3 rotating buffer sets (> last-level cache)."""
-    n, kl, ku = 1 << 20, 8, 8
-    rng = np.random.default_rng(1)
-    sets = []
-    for _ in range(3):
-        A = bh.BandedMatrix.from_host(np.asfortranarray(rng.standard_normal((kl + ku + 1, n))), n, kl, ku)
-        x = bh.DeviceArray.from_numpy(rng.standard_normal(n))
-        y = bh.DeviceArray((n,), np.float64)
-        sets.append((A, x, y))
-    for i in range(warmup * 3):
-        A, x, y = sets[i % 3]
-        bh.mul(y, A, x)
-    bh.sync()
-    reps = max(30, steps * 6)
+def gpu_time(bh, fn, reps):
     t = bh.Timer()
     t.start()
-    for i in range(reps):
-        A, x, y = sets[i % 3]
-        bh.mul(y, A, x)
+    for _ in range(reps):
+        fn()
     t.stop()
-    ms = t.elapsed_ms() / reps
-    nbytes = (kl + ku + 1 + 2) * 8.0 * n
-    return nbytes / (ms * 1e-3) / 1e9, ms, nbytes
+    return t.elapsed_ms() / reps
+
+
+# ----------------------------------------------------------------- precision (headline)
+def bench_precision(bh, steps, warmup, shift=True):
+    import ctypes as C
+
+    from banded_hip import _lib
+
+    n, l, u, nrhs = 65536, 128, 128, 16
+    A_h, B_h = make_precision_problem(n, l, u, nrhs, shift=shift)
+    A = bh.BandedMatrix.from_host(A_h, n, l, u)
+    B = bh.DeviceArray.from_numpy(B_h)
+    X = bh.DeviceArray((n, nrhs), np.float64)
+    solver = bh.MixedSolver(n, l, u, nrhs)
+    for _ in range(warmup):
+        res = solver.solve(A, B, X)
+    bh.sync()
+    iters = []
+    t = bh.Timer()
+    t.start()
+    for _ in range(steps):
+        res = solver.solve(A, B, X)
+        iters.append(res.iters)
+    t.stop()
+    ms_mixed = t.elapsed_ms() / steps
+    x_mixed = X.numpy()
+
+    def fp64_solve():
+        F = bh.lu(A)
+        Xd = B.copy()
+        bh.ldiv_(F, Xd)
+        return F, Xd
+
+    for _ in range(warmup):
+        F64, Xd = fp64_solve()
+    bh.sync()
+    t.start()
+    for _ in range(steps):
+        F64, Xd = fp64_solve()
+    t.stop()
+    ms_fp64 = t.elapsed_ms() / steps
+    x_fp64 = Xd.numpy()
+
+    # the pieces of the mixed step, each timed alone on the library stream
+    ab32 = bh.pad_for_lu(A, np.float32)
+    keep = ab32.data.copy()
+    ipiv = bh.DeviceArray((n,), np.int64)
+    info = C.c_int64()
+    lu_ms = []
+    for _ in range(max(3, steps)):
+        _lib.call("bh_memcpy_dtod", ab32.data.ptr, keep.ptr, keep.nbytes)
+        t.start()
+        _lib.call("bh_sgbtrf", n, n, l, u, ab32.data.ptr, ab32.data.ld, ipiv.ptr, C.byref(info))
+        t.stop()
+        lu_ms.append(t.elapsed_ms())
+    B32 = bh.DeviceArray.from_numpy(np.asfortranarray(B_h.astype(np.float32)))
+    trs_ms = []
+    for _ in range(max(3, steps)):
+        t.start()
+        _lib.call("bh_sgbtrs", b"N", n, l, u, nrhs, ab32.data.ptr, ab32.data.ld, ipiv.ptr, B32.ptr, n)
+        t.stop()
+        trs_ms.append(t.elapsed_ms())
+    return dict(n=n, l=l, u=u, nrhs=nrhs, A_h=A_h, B_h=B_h, ms_mixed=ms_mixed, ms_fp64=ms_fp64, iters=iters,
+                x_mixed=x_mixed, x_fp64=x_fp64, ipiv64=F64.ipiv, lu32_ms=float(np.median(lu_ms[1:])),
+                trs32_ms=float(np.median(trs_ms[1:])))
+
+
+# ----------------------------------------------------------------- bandwidth-bound kernels
+def bench_bandwidth(bh, steps):
+    n = 1 << 20
+    rng = np.random.default_rng(1)
+    out = {}
+    for dt, tag in ((np.float64, "fp64"), (np.float32, "fp32")):
+        kl = ku = 8
+        sets = []
+        for _ in range(3):
+            A = bh.BandedMatrix.from_host(np.asfortranarray(rng.standard_normal((17, n)).astype(dt)), n, kl, ku)
+            x = bh.DeviceArray.from_numpy(rng.standard_normal(n).astype(dt))
+            y = bh.DeviceArray((n,), dt)
+            sets.append((A, x, y))
+        k = [0]
+
+        def one():
+            A, x, y = sets[k[0] % 3]
+            k[0] += 1
+            bh.mul(y, A, x)
+
+        for _ in range(6):
+            one()
+        bh.sync()
+        ms = gpu_time(bh, one, max(30, steps * 6))
+        nbytes = (17 + 2) * np.dtype(dt).itemsize * float(n)
+        out[f"gbmv_{tag}_8x8_n2p20"] = {"GBps": nbytes / ms / 1e6, "frac_of_hbm_peak": nbytes / ms / 1e6 / HBM_PEAK_GBPS,
+                                        "us": ms * 1e3, "bytes": nbytes}
+        if dt == np.float64:
+            # 64 right-hand sides: A read once, X and Y streamed
+            X = bh.DeviceArray.from_numpy(np.asfortranarray(rng.standard_normal((n, 64))))
+            Y = bh.DeviceArray((n, 64), np.float64)
+            A0 = sets[0][0]
+            f = lambda: bh.mul(Y, A0, X)
+            f()
+            bh.sync()
+            ms = gpu_time(bh, f, max(5, steps))
+            nbytes = (17 + 64 + 64) * 8.0 * n
+            out["gbmv_fp64_8x8_64rhs_n2p20"] = {"GBps": nbytes / ms / 1e6, "frac_of_hbm_peak": nbytes / ms / 1e6 / HBM_PEAK_GBPS,
+                                                "us": ms * 1e3, "bytes": nbytes}
+            # A*B: (8,8) x (4,4) -> (12,12)
+            Bm = bh.BandedMatrix.from_host(np.asfortranarray(rng.standard_normal((9, n))), n, 4, 4)
+            Cm = bh.BandedMatrix.zeros((n, n), (12, 12))
+            g = lambda: bh.gbmm(A0, Bm, Cm)
+            g()
+            bh.sync()
+            ms = gpu_time(bh, g, max(5, steps))
+            nbytes = (17 + 9 + 25) * 8.0 * n
+            out["gbmm_fp64_8x8_4x4_n2p20"] = {"GBps": nbytes / ms / 1e6, "frac_of_hbm_peak": nbytes / ms / 1e6 / HBM_PEAK_GBPS,
+                                              "us": ms * 1e3, "bytes": nbytes}
+    return out
+
+
+# ----------------------------------------------------------------- batching
+def bench_batched(bh, skip_cpu):
+    n, kl, ku, bands, rhs = bh.c4_batch(4096, 0)
+    out = {"systems": 4096, "rows_total": int(n.sum())}
+    for dt, tag in ((np.float64, "fp64"), (np.float32, "fp32")):
+        bt = bh.BatchedGBSV(n, kl, ku, dt)
+        AB, B = bt.pack([b.astype(dt) for b in bands], [r.astype(dt) for r in rhs])
+        bt.upload(AB, B)
+        tuned_ms = bt.autotune()
+        g = bt.graph()
+        ms = []
+        t = bh.Timer()
+        for _ in range(3):
+            bt.upload(AB, B)
+            bh.sync()
+            t.start()
+            g.launch()
+            t.stop()
+            ms.append(t.elapsed_ms())
+        g.close()
+        out[tag] = {"ms_per_batch": float(np.median(ms)), "systems_per_s": 4096 / (np.median(ms) * 1e-3),
+                    "schedule": list(bt.schedule()), "autotune_best_ms": tuned_ms}
+        bt.close()
+    if not skip_cpu:
+        import scipy.linalg.lapack as la
+
+        sample = range(0, 4096, 16)  # 256 systems, scaled to the batch
+        t0 = time.perf_counter()
+        for s in sample:
+            l, u = int(kl[s]), int(ku[s])
+            ab = np.zeros((2 * l + u + 1, n[s]), order="F")
+            ab[l:, :] = bands[s]
+            lub, piv, info = la.dgbtrf(ab, l, u)
+            la.dgbtrs(lub, l, u, rhs[s], piv)
+        tc = (time.perf_counter() - t0) * 4096 / len(sample)
+        out["cpu_1core_ms_per_batch_est"] = tc * 1e3
+        out["cpu_sample"] = "every 16th system (256) with scipy OpenBLAS dgbtrf+dgbtrs on 1 core, scaled x16"
+    return out
+
+
+# ----------------------------------------------------------------- differentiation
+def bench_adjoint(bh, steps, skip_cpu):
+    n, l, u = 262144, 16, 16
+    rng = np.random.default_rng(4)
+    A = np.asfortranarray(rng.standard_normal((l + u + 1, n)))
+    A[u, :] += 2 * (l + u)
+    b, w, x0 = (rng.standard_normal(n) for _ in range(3))
+    dA = bh.BandedMatrix.from_host(A, n, l, u)
+    db, dw, dx0 = (bh.DeviceArray.from_numpy(v) for v in (b, w, x0))
+    bh.loss_and_grad(dA, db, dw, dx0)
+    bh.sync()
+    ms = gpu_time(bh, lambda: bh.loss_and_grad(dA, db, dw, dx0), max(2, steps // 2))
+    out = {"n": n, "l": l, "u": u, "ms_loss_and_grad": ms}
+    if not skip_cpu:
+        import scipy.linalg.lapack as la
+
+        t0 = time.perf_counter()
+        ab = np.zeros((2 * l + u + 1, n), order="F")
+        ab[l:, :] = A
+        lub, piv, info = la.dgbtrf(ab, l, u)
+        x, _ = la.dgbtrs(lub, l, u, b, piv)
+        lam, _ = la.dgbtrs(lub, l, u, w, piv, trans=1)
+        out["cpu_ms_loss_and_grad_lapack_part"] = (time.perf_counter() - t0) * 1e3
+    return out
+
+
+# ----------------------------------------------------------------- plain, CPU-runnable case
+def bench_plain(bh, steps, skip_cpu):
+    n, l, u = 10000, 4, 3
+    rng = np.random.default_rng(0)
+    A_h = np.asfortranarray(rng.random((l + u + 1, n)))
+    b_h = rng.standard_normal(n)
+    A = bh.BandedMatrix.from_host(A_h, n, l, u)
+    b = bh.DeviceArray.from_numpy(b_h)
+    y = bh.DeviceArray((n,), np.float64)
+    C = bh.BandedMatrix.zeros((n, n), (8, 6))
+    f_mv = lambda: bh.mul(y, A, b)
+    f_mm = lambda: bh.gbmm(A, A, C)
+    f_sv = lambda: bh.solve(A, b)
+    for f in (f_mv, f_mm, f_sv):
+        f()
+    bh.sync()
+    out = {"n": n, "l": l, "u": u, "us_gbmv": gpu_time(bh, f_mv, 50) * 1e3, "us_gbmm": gpu_time(bh, f_mm, 50) * 1e3,
+           "us_solve": gpu_time(bh, f_sv, 5) * 1e3}
+    if not skip_cpu:
+        import scipy.linalg.lapack as la
+
+        ab = np.zeros((2 * l + u + 1, n), order="F")
+        ab[l:, :] = A_h
+        t0 = time.perf_counter()
+        for _ in range(20):
+            lub, piv, info = la.dgbtrf(ab, l, u)
+            la.dgbtrs(lub, l, u, b_h, piv)
+        out["cpu_us_solve_1core"] = (time.perf_counter() - t0) / 20 * 1e6
+    return out
+
+
+def traffic_probe(bh, name):
+    """A few launches of one kernel plus a 256 MiB device copy, for counter runs."""
+    import ctypes as C
+
+    from banded_hip import _lib
+
+    src = bh.DeviceArray((1 << 25,), np.float64)
+    dst = bh.DeviceArray((1 << 25,), np.float64)
+    for _ in range(3):
+        _lib.call("bh_memcpy_dtod", dst.ptr, src.ptr, src.nbytes)
+    if name == "gbmv64":
+        n = 1 << 20
+        A = bh.BandedMatrix.from_host(np.asfortranarray(np.random.default_rng(1).standard_normal((17, n))), n, 8, 8)
+        x = bh.DeviceArray.from_numpy(np.ones(n))
+        y = bh.DeviceArray((n,), np.float64)
+        for _ in range(3):
+            bh.mul(y, A, x)
+    elif name == "lu32":
+        n, l, u = 65536, 128, 128
+        A_h, _ = make_precision_problem(n, l, u, 1)
+        ab = bh.pad_for_lu(bh.BandedMatrix.from_host(A_h, n, l, u), np.float32)
+        ipiv = bh.DeviceArray((n,), np.int64)
+        info = C.c_int64()
+        _lib.call("bh_sgbtrf", n, n, l, u, ab.data.ptr, ab.data.ld, ipiv.ptr, C.byref(info))
+    bh.sync()
+    print(json.dumps({"traffic_probe": name, "copy_bytes": src.nbytes}))
 
 
 def main():
@@ -98,7 +338,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--noshift", action="store_true", help="the pivoting variant (no diagonal shift)")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--headline-only", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--traffic-probe", default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -110,97 +352,61 @@ def main():
     L = bh.load_library()
     if L.bh_init(0) != 0:
         raise RuntimeError(L.bh_last_error().decode())
+    if args.traffic_probe:
+        traffic_probe(bh, args.traffic_probe)
+        return
 
-    n, l, u, nrhs = 65536, 128, 128, 16
-    A_h, B_h = make_precision_problem(n, l, u, nrhs, shift=not args.noshift)
-    A = bh.BandedMatrix.from_host(A_h, n, l, u)
-    B = bh.DeviceArray.from_numpy(B_h)
-    X = bh.DeviceArray((n, nrhs), np.float64)
-    solver = bh.MixedSolver(n, l, u, nrhs)
+    log("precision workload")
+    P = bench_precision(bh, args.steps, args.warmup, shift=not args.noshift)
+    n, l, u, nrhs = P["n"], P["l"], P["u"], P["nrhs"]
+    extra = {}
+    if not args.headline_only:
+        log("bandwidth-bound kernels")
+        extra.update(bench_bandwidth(bh, args.steps))
+        log("batched systems")
+        extra["batched_4096"] = bench_batched(bh, args.skip_cpu)
+        log("differentiation")
+        extra["adjoint_n262144_16x16"] = bench_adjoint(bh, args.steps, args.skip_cpu)
+        log("plain case")
+        extra["plain_n1e4_4x3"] = bench_plain(bh, args.steps, args.skip_cpu)
+    else:
+        extra.update(bench_bandwidth(bh, args.steps))
 
-    # --- mixed precision A\b (the headline)
-    for _ in range(args.warmup):
-        res = solver.solve(A, B, X)
-    bh.sync()
-    t = bh.Timer()
-    t.start()
-    iters = []
-    for _ in range(args.steps):
-        res = solver.solve(A, B, X)
-        iters.append(res.iters)
-    t.stop()
-    ms_mixed = t.elapsed_ms() / args.steps
-    x_mixed = X.numpy()
-
-    # --- plain FP64 GPU A\b (factor + solve), same inputs
-    def fp64_solve():
-        F = bh.lu(A)
-        Xd = B.copy()
-        bh.ldiv_(F, Xd)
-        return F, Xd
-
-    for _ in range(args.warmup):
-        F64, Xd = fp64_solve()
-    bh.sync()
-    t.start()
-    for _ in range(args.steps):
-        F64, Xd = fp64_solve()
-    t.stop()
-    ms_fp64 = t.elapsed_ms() / args.steps
-    x_fp64 = Xd.numpy()
-
-    # --- dominant kernel of the mixed path: the FP32 band LU, timed alone
-    import ctypes as C
-
-    from banded_hip import _lib
-
-    ab32 = bh.pad_for_lu(A, np.float32)
-    ab_keep = ab32.data.copy()
-    ipiv = bh.DeviceArray((n,), np.int64)
-    info = C.c_int64()
-    lu_ms = []
-    for _ in range(max(2, args.steps)):
-        # restore the padded FP32 copy, then time one factorisation on the library stream
-        _lib.call("bh_memcpy_dtod", ab32.data.ptr, ab_keep.ptr, ab_keep.nbytes)
-        t.start()
-        _lib.call("bh_sgbtrf", n, n, l, u, ab32.data.ptr, ab32.data.ld, ipiv.ptr, C.byref(info))
-        t.stop()
-        lu_ms.append(t.elapsed_ms())
-    lu_ms_avg = float(np.mean(lu_ms[1:]))
-    lu_fl = lu_flops(n, l, u)
-
-    # --- gbmv HBM roofline (second half of the metric)
-    gbmv_gbps, gbmv_ms, gbmv_bytes = bench_gbmv(bh, args.steps, args.warmup)
-
-    # --- CPU baseline + FP64 oracle solution
     out = {}
     if not args.skip_cpu:
+        log("CPU baseline")
         threads = args.cpu_threads or os.cpu_count() or 1
-        xs, tcpu = [], []
+        tcpu = []
         for _ in range(3):
-            xc, pivc, tc = cpu_reference_solve(A_h, B_h, l, u, threads)
+            xc, pivc, tc = cpu_reference_solve(P["A_h"], P["B_h"], l, u, threads)
             tcpu.append(tc)
-        xref = xc
-        fe_mixed = float(np.abs(x_mixed - xref).max() / np.abs(xref).max())
-        fe_fp64 = float(np.abs(x_fp64 - xref).max() / np.abs(xref).max())
-        piv_match = int(np.sum(F64.ipiv == pivc))
-        piv32 = res.ipiv32 if res.ipiv32.size else None
+        fe = lambda x: float(np.abs(x - xc).max() / np.abs(xc).max())
         out["cpu_baseline"] = {
             "value": 1.0 / float(np.median(tcpu)), "unit": "solves/s", "cores": threads, "kind": "reference",
-            "sample": ("3 full A\\b solves (n=65536, l=u=128, 16 RHS) with OpenBLAS 0.3.28 dgbtrf+dgbtrs via scipy "
-                       "-- the LAPACK routines BandedLU.jl:98 / linalg.jl:28 call; median reported"),
+            "sample": ("3 full A\\b solves of the same workload (n=65536, l=u=128, 16 RHS) with OpenBLAS 0.3.28 "
+                       "dgbtrf+dgbtrs via scipy -- the LAPACK routines BandedLU.jl:98 / linalg.jl:28 call; median"),
         }
-        out["forward_error_vs_cpu_fp64"] = {"mixed": fe_mixed, "fp64_gpu": fe_fp64}
-        out["pivots_fp64_gpu_equal_cpu"] = f"{piv_match}/{n}"
-    value = 1000.0 / ms_mixed
+        out["forward_error_vs_cpu_fp64"] = {"mixed": fe(P["x_mixed"]), "fp64_gpu": fe(P["x_fp64"])}
+        out["pivots_fp64_gpu_equal_cpu"] = f"{int(np.sum(P['ipiv64'] == pivc))}/{n}"
+
+    g = extra["gbmv_fp64_8x8_n2p20"]
+    lu_fl = lu_flops(n, l, u)
+    lu_tf = lu_fl / (P["lu32_ms"] * 1e-3) / 1e12
+    trs_fl = solve_flops(n, l, u, nrhs)
+    trs_tf = trs_fl / (P["trs32_ms"] * 1e-3) / 1e12
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("lu32_bytes_per_launch")
     line = {
         "metric": "A\\b solves/s n=65536 l=u=128 mixed vs FP64 at 1e-12 err; gbmv HBM GB/s",
-        "value": value,
+        "value": 1000.0 / P["ms_mixed"],
         "unit": "solves/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_mixed,
+        "ms_per_step": P["ms_mixed"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -209,26 +415,24 @@ def main():
                 "synthetic (seeded N(0,1) band, no shift; 16 RHS N(0,1))",
         "config": {"workload": "precision: A\\b n=65536 l=u=128 16 RHS" + (" (no shift)" if args.noshift else ""),
                    "n": n, "l": l, "u": u, "nrhs": nrhs},
-        "fp64_gpu_solves_per_s": 1000.0 / ms_fp64,
-        "ms_per_step_fp64_gpu": ms_fp64,
-        "refinement_iters": iters,
-        "gbmv_fp64_8x8_n2p20": {"GBps": gbmv_gbps, "frac_of_hbm_peak": gbmv_gbps / HBM_PEAK_GBPS, "us": gbmv_ms * 1e3,
-                                "bytes": gbmv_bytes},
+        "fp64_gpu_solves_per_s": 1000.0 / P["ms_fp64"],
+        "ms_per_step_fp64_gpu": P["ms_fp64"],
+        "refinement_iters": P["iters"],
+        "gbmv_GBps": g["GBps"],
         "roofline": {
-            "kernel": "reg_gbtf2_kernel<float,16,32,9,8> (FP32 band LU, the dominant kernel of the mixed A\\b)",
-            "bound": "mfma",
-            "achieved": lu_fl / (lu_ms_avg * 1e-3) / 1e12,
-            "peak": FP32_PEAK_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": lu_fl / (lu_ms_avg * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
-            "traffic": None,
-            "kernel_ms": lu_ms_avg,
+            "kernel": "gbtrf_blk_kernel<float,15,10,4> (FP32 band LU, blocked; the largest kernel of the mixed A\\b)",
+            "bound": "mfma", "achieved": lu_tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": lu_tf / FP32_PEAK_TFLOPS, "traffic": traffic, "kernel_ms": P["lu32_ms"],
             "flops_per_launch": lu_fl,
         },
-        "roofline_gbmv": {"bound": "hbm", "achieved": gbmv_gbps, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                          "frac": gbmv_gbps / HBM_PEAK_GBPS, "traffic": None},
+        "roofline_solve": {"kernel": "gbtrs_n_ring<float,3,5,32> (16 RHS, FP32)", "bound": "mfma", "achieved": trs_tf,
+                           "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": trs_tf / FP32_PEAK_TFLOPS,
+                           "kernel_ms": P["trs32_ms"], "flops_per_launch": trs_fl},
+        "roofline_gbmv": {"bound": "hbm", "achieved": g["GBps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                          "frac": g["GBps"] / HBM_PEAK_GBPS, "traffic": None},
     }
     line.update(out)
+    line["workloads"] = extra
     print(json.dumps(line))
 
 

@@ -204,6 +204,28 @@ int bh_sgbsv_batched_graph(bh_batch_plan *plan, float *AB, float *B, bh_int *ipi
 int bh_batch_graph_launch(bh_batch_graph *graph);
 int bh_batch_graph_destroy(bh_batch_graph *graph);
 
+/* ------------------------------------------------------------- adjoint */
+/* Gradient of L = w' A^{-1} b + 1/2 ||A x0||^2 with respect to A's band, given
+ * lam = A^{-T} w (bh_?gbtrs 'T' with the same factors), x = A \ b, y0 = A x0:
+ * G_ij = fma(y0_i, x0_j, -(lam_i x_j)) for every (i,j) of the m x n band
+ * (G is band storage, ldg >= kl+ku+1; slots outside the matrix untouched).
+ * No reference counterpart (the reference has no rrule); the adjoint solve
+ * reuses the transposed LU solve of src/banded/linalg.jl:41-47. */
+int bh_dgb_band_grad(bh_int m, bh_int n, bh_int kl, bh_int ku, const double *lam, const double *x,
+                     const double *y0, const double *x0, double *G, bh_int ldg);
+int bh_sgb_band_grad(bh_int m, bh_int n, bh_int kl, bh_int ku, const float *lam, const float *x,
+                     const float *y0, const float *x0, float *G, bh_int ldg);
+/* G <- beta G + P(alpha1 u1 v1' + alpha2 u2 v2'), P = restriction to the band,
+ * each term one fma(alpha u_i, v_j, g) in that order; a NULL (u, v) pair is
+ * skipped.  The pullbacks of A \ b (alpha = -1, u = A^{-T} xbar, v = x) and of
+ * A * x0 (alpha = 1, u = ybar, v = x0) in band storage. */
+int bh_dgb_band_outer(bh_int m, bh_int n, bh_int kl, bh_int ku, double alpha1, const double *u1,
+                      const double *v1, double alpha2, const double *u2, const double *v2, double beta,
+                      double *G, bh_int ldg);
+int bh_sgb_band_outer(bh_int m, bh_int n, bh_int kl, bh_int ku, float alpha1, const float *u1,
+                      const float *v1, float alpha2, const float *u2, const float *v2, float beta,
+                      float *G, bh_int ldg);
+
 /* ------------------------------------------------------- mixed precision */
 /* X = A \ B for an FP64 band matrix A ((kl+ku+1) x n, lda) via an FP32 band LU
  * and FP64 iterative refinement: residuals B - A X in FP64, corrections from
