@@ -189,6 +189,10 @@ __global__ void scale_vec(i64 len, i64 nrhs, T beta, T *__restrict__ Y, i64 incy
 }
 
 template <typename T>
+int slab_gbmv_n(i64 m, i64 n, i64 kl, i64 ku, i64 nrhs, T alpha, const T *A, i64 lda, const T *X, i64 incx, i64 ldx,
+                T beta, T *Y, i64 incy, i64 ldy, hipStream_t st);
+
+template <typename T>
 static int gbmv_impl(char trans, i64 m, i64 n, i64 kl, i64 ku, i64 nrhs, T alpha, const T *A, i64 lda,
                      const T *X, i64 incx, i64 ldx, T beta, T *Y, i64 incy, i64 ldy) {
     const bool nt = (trans == 'N' || trans == 'n');
@@ -200,6 +204,10 @@ static int gbmv_impl(char trans, i64 m, i64 n, i64 kl, i64 ku, i64 nrhs, T alpha
         hipLaunchKernelGGL(scale_vec<T>, dim3((unsigned)cdiv(tot, 256)), dim3(256), 0, s, leny, nrhs, beta, Y, incy, ldy);
         BH_LAUNCH_CHECK();
         return BH_OK;
+    }
+    if (nt) {
+        const int st = slab_gbmv_n<T>(m, n, kl, ku, nrhs, alpha, A, lda, X, incx, ldx, beta, Y, incy, ldy, s);
+        if (st != -1) return st;
     }
     const i64 R = kTileThreads;
     const size_t lds = tile_lds_bytes<T>(R, kl, ku, 1);

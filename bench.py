@@ -162,7 +162,9 @@ def bench_bandwidth(bh, steps):
     for dt, tag in ((np.float64, "fp64"), (np.float32, "fp32")):
         kl = ku = 8
         sets = []
-        for _ in range(3):
+        # enough matrices that the rotation spans > 512 MB: no launch finds
+        # its operands in the 256 MB last-level cache
+        for _ in range(4 if dt == np.float64 else 7):
             A = bh.BandedMatrix.from_host(np.asfortranarray(rng.standard_normal((17, n)).astype(dt)), n, kl, ku)
             x = bh.DeviceArray.from_numpy(rng.standard_normal(n).astype(dt))
             y = bh.DeviceArray((n,), dt)
@@ -170,7 +172,7 @@ def bench_bandwidth(bh, steps):
         k = [0]
 
         def one():
-            A, x, y = sets[k[0] % 3]
+            A, x, y = sets[k[0] % len(sets)]
             k[0] += 1
             bh.mul(y, A, x)
 

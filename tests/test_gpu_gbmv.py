@@ -199,3 +199,34 @@ def test_gbmm_wider_c_and_banderror(bh, oracle):
     with pytest.raises(bh.BandError):
         bh.gbmm(bh.BandedMatrix.from_host(A2, 10, 1, 1), bh.BandedMatrix.from_host(B2, 10, 1, 1),
                 bh.BandedMatrix.from_host(C2, 10, 1, 1))
+
+
+# The persistent wave-pipelined single-vector kernel (beta = 0, m <= n,
+# kl + ku <= 64, contiguous aligned storage): several tiles per wave, the
+# XCD split with fewer tiles than waves, the widest band it takes, and each
+# pipeline depth.  BANDEDHIP_GBMV_WAVE picks the depth (1 -> 2 stages).
+WAVE_CASES = [
+    (1 << 18, 1 << 18, 8, 8),
+    (200003, 200003, 40, 24),
+    (100000, 100001, 32, 32),
+    (64 * 8 * 3 + 5, 64 * 8 * 3 + 5, 1, 0),
+    (99999, 150000, 5, 7),
+    (70, 70, 3, 3),
+]
+
+
+@pytest.mark.parametrize("stages", ["1", "3", "4"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("m,n,kl,ku", WAVE_CASES)
+def test_gbmv_wave_pipeline_bitexact(bh, oracle, monkeypatch, stages, dtype, m, n, kl, ku):
+    monkeypatch.setenv("BANDEDHIP_GBMV_WAVE", stages)
+    rng = np.random.default_rng(m + 31 * kl + ku)
+    A = band(rng, kl, ku, n, dtype, dist="n", nan_corners=True, m=m)
+    x = rng.standard_normal(n).astype(dtype)
+    y = np.full(m, np.nan, dtype)
+    ref = oracle.gbmv("N", m, n, kl, ku, dtype(-0.75), A, x, dtype(0.0), y)
+    BA = bh.BandedMatrix.from_host(A, m, kl, ku)
+    dx, dy = bh.DeviceArray.from_numpy(x), bh.DeviceArray.from_numpy(y)
+    bh.mul(dy, BA, dx, -0.75, 0.0)
+    out = dy.numpy()
+    assert bits_equal(out, ref), np.abs(out - ref).max()
