@@ -21,11 +21,13 @@
 // systems in one persistent launch; each wave walks its share of the
 // schedule).
 #include "bh_common.hpp"
+#include "bh_lu_lane.hpp"
 
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
+#include <type_traits>
 
 namespace bh {
 
@@ -251,66 +253,7 @@ __global__ void __launch_bounds__(64) gbtrf_small_kernel(int m, int n, int kl, i
 }
 
 // ---------------------------------------------------------------- batched
-struct BatchDesc {
-    i64 ab_off;   // element offset of the system's padded AB
-    i64 vec_off;  // element offset of its b / ipiv
-    int n, kl, ku, cls, sys;
-};
-
-// One body per register class (inlined; the kernel's register budget is
-// that of the largest class).
-template <typename T, int RL>
-__device__ __forceinline__ int batch_sys(int n, int kl, int ku, T *ab, i64 ld, i64 *ip, T *b) {
-    return small_gbsv_wave<T, RL, true>(n, n, kl, ku, ab, ld, ip, b);
-}
-
-template <typename T>
-__device__ __forceinline__ int batch_one(const BatchDesc &d, T *AB, T *Bv, i64 *IP) {
-    const i64 ld = 2 * d.kl + d.ku + 1;
-    T *ab = AB + d.ab_off;
-    T *b = Bv + d.vec_off;
-    i64 *ip = IP + d.vec_off;
-    switch (d.cls) {
-#define BH_BATCH_CASE(C, RLv) \
-    case C:                   \
-        return batch_sys<T, RLv>(d.n, d.kl, d.ku, ab, ld, ip, b);
-        BH_BATCH_CASE(0, 2)
-        BH_BATCH_CASE(1, 3)
-        BH_BATCH_CASE(2, 5)
-        default:
-            return batch_sys<T, 9>(d.n, d.kl, d.ku, ab, ld, ip, b);
-#undef BH_BATCH_CASE
-    }
-}
-
-// Persistent: wave w of the W resident waves solves systems w, w + W,
-// w + 2W, ... of the schedule (the plan orders the systems so that this
-// visit order balances the work).  All loop bounds are wave-uniform
-// (readfirstlane), so the step loops stay scalar loops.
-template <typename T>
-__global__ void __launch_bounds__(256) gbsv_batched_kernel(const BatchDesc *__restrict__ desc, int nsys, T *AB,
-                                                           T *Bv, i64 *IP, i64 *info) {
-    auto uni64 = [](i64 v) {
-        const int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffff));
-        const int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
-        return (i64)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
-    };
-    const int nw = (int)(gridDim.x * (blockDim.x >> 6));
-    const int w0 = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)));
-    for (int q = w0; q < nsys; q += nw) {
-        const BatchDesc dv = desc[q];
-        BatchDesc d;
-        d.ab_off = uni64(dv.ab_off);
-        d.vec_off = uni64(dv.vec_off);
-        d.n = __builtin_amdgcn_readfirstlane(dv.n);
-        d.kl = __builtin_amdgcn_readfirstlane(dv.kl);
-        d.ku = __builtin_amdgcn_readfirstlane(dv.ku);
-        d.cls = __builtin_amdgcn_readfirstlane(dv.cls);
-        d.sys = __builtin_amdgcn_readfirstlane(dv.sys);
-        const int r = batch_one<T>(d, AB, Bv, IP);
-        if (d.sys >= 0 && __lane_id() == 0) info[d.sys] = r;
-    }
-}
+// (the batched kernel and its descriptor live in bh_lu_lane.hpp)
 
 // Register-row classes: kl + 1 <= RL for RL in {2, 3, 5, 9, 12, 17, 24, 32}.
 constexpr int kSmallRL[] = {2, 3, 5, 9, 12, 17, 24, 32};
@@ -322,13 +265,21 @@ int small_class(i64 kl) {
 
 }  // namespace
 
-// Single-system narrow-band LU (kl + ku <= 63, kl <= 31), or -1.
+// Single-system narrow-band LU (kl + ku <= 63, kl <= 31), or -1.  The
+// rotating-register kernel runs for kl <= 8 and kl = 16 when the band
+// storage is under 2 GiB (its buffer offsets are 32-bit); other widths, and
+// BANDEDHIP_SMALL_LU=class, take the register-class kernel.
 template <typename T>
 int small_gbtrf_launch(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, i64 *dinfo, hipStream_t st) {
     if (kl + ku > 63 || m >= (1ll << 30) || n >= (1ll << 30)) return -1;
     const int cls = small_class(kl);
     if (cls < 0) return -1;
     const int mi = (int)m, ni = (int)n, li = (int)kl, ui = (int)ku;
+    const char *e = getenv("BANDEDHIP_SMALL_LU");
+    if (!(e && strcmp(e, "class") == 0)) {
+        const int r = lane_gbtrf_launch(m, n, kl, ku, ab, ldab, ipiv, dinfo, st);
+        if (r != -1) return r;
+    }
 #define BH_SMALL_CASE(C, RLv)                                                                                  \
     case C:                                                                                                    \
         hipLaunchKernelGGL((gbtrf_small_kernel<T, RLv>), dim3(1), dim3(64), 0, st, mi, ni, li, ui, ab, ldab, ipiv, \
@@ -424,8 +375,9 @@ int plan_set_order(bh_batch_plan *p, int kind) {
 template <typename T>
 int batch_launch(bh_batch_plan *p, T *AB, T *B, i64 *ipiv, i64 *info, hipStream_t st) {
     if (p->nsys == 0) return BH_OK;
-    hipLaunchKernelGGL(gbsv_batched_kernel<T>, dim3((unsigned)p->blocks), dim3(64 * p->waves_per_block), 0, st,
-                       p->desc, (int)p->desc_len, AB, B, ipiv, info);
+    const int r = lane_batch_launch((unsigned)p->blocks, (unsigned)(64 * p->waves_per_block), st, p->desc,
+                                    (int)p->desc_len, AB, B, ipiv, info);
+    if (r != BH_OK) return r;
     BH_LAUNCH_CHECK();
     return BH_OK;
 }
@@ -554,9 +506,9 @@ int bh_batch_plan_create(bh_int nsys, const bh_int *n, const bh_int *kl, const b
     }
     for (i64 s = 0; s < nsys; ++s) {
         if (p->n[s] < 0 || p->n[s] >= (1ll << 30) || p->kl[s] < 0 || p->ku[s] < 0 || p->kl[s] + p->ku[s] > 63 ||
-            p->kl[s] > 8) {
+            p->kl[s] > 8 || (2 * p->kl[s] + p->ku[s] + 1) * p->n[s] * 8 >= (1ll << 31)) {
             set_error("bh_batch_plan_create: system %lld (n=%lld kl=%lld ku=%lld) is outside the batched kernel "
-                      "(kl <= 8, kl+ku <= 63)",
+                      "(kl <= 8, kl+ku <= 63, band storage under 2 GiB)",
                       (long long)s, (long long)p->n[s], (long long)p->kl[s], (long long)p->ku[s]);
             delete p;
             return -2;
