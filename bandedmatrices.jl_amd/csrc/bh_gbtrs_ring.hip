@@ -42,25 +42,45 @@ struct Unroll<0> {
 
 __device__ __forceinline__ int mini(int a, int b) { return a < b ? a : b; }
 
-// Wave-uniform register index (scalar branches).
-template <int N, typename T>
-__device__ __forceinline__ T reg_pick(const T (&w)[N], int k) {
-    T r = w[0];
-#pragma unroll
-    for (int i = 1; i < N; ++i)
-        if (k == i) r = w[i];
-    return r;
-}
+constexpr int kRingWord3 = 0x00020000;  // word 3 of a raw buffer resource (gfx9 family)
+// A byte offset past the end of any AB the ring kernel takes (those are under
+// 2 GiB - 16 MiB), with room for +-64 * 5 entries either side.
+constexpr int kRingPastEnd = 0x7f800000;
 
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t ring_rsrc(const void *p, i64 bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int)bytes, kRingWord3);
+}
+__device__ __forceinline__ double ring_ld(__amdgpu_buffer_rsrc_t r, int off, double) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ float ring_ld(__amdgpu_buffer_rsrc_t r, int off, float) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+__device__ __forceinline__ void ring_st(__amdgpu_buffer_rsrc_t r, int off, double v) {
+    typedef unsigned v2u __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, v), r, off, 0, 0);
+}
+__device__ __forceinline__ void ring_st(__amdgpu_buffer_rsrc_t r, int off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+}
 // NF = 1 + ceil(kl/64) forward registers, NB = 1 + ceil((kl+ku)/64) backward
-// registers, PFD = prefetch distance in steps (divides 64).
+// registers, PFD = prefetch distance in steps (divides 64).  The window
+// registers are only ever indexed with compile-time indices (an interchange
+// selects with uniform conditions and replaces single lanes), and all memory
+// goes through buffer resources: a lane with nothing to load passes an
+// offset past the end and reads 0, so no access sits in a branch.
 template <typename T, int NF, int NB, int PFD>
 __global__ void __launch_bounds__(64) gbtrs_n_ring(int n, int kl, int ku, const T *__restrict__ ab, i64 ld,
                                                    const i64 *__restrict__ ipiv, T *B, i64 ldb) {
     static_assert(64 % PFD == 0, "prefetch distance divides the block");
+    constexpr int SZ = (int)sizeof(T);
     const int lane = threadIdx.x;
     T *b = B + (i64)blockIdx.x * ldb;
     const int kd = kl + ku;
+    const __amdgpu_buffer_rsrc_t rab = ring_rsrc(ab, ld * (i64)n * SZ);
+    const __amdgpu_buffer_rsrc_t rb = ring_rsrc(b, (i64)n * SZ);
+    const __amdgpu_buffer_rsrc_t rip = ring_rsrc(ipiv, (i64)n * 8);
+    auto bload = [&](int e) { return ring_ld(rb, (e >= 0 && e < n) ? e * SZ : -1, T()); };
 
     // ------------------------------------------------------------ forward: L and pivots
     if (kl > 0 && n > 1) {
@@ -68,31 +88,33 @@ __global__ void __launch_bounds__(64) gbtrs_n_ring(int n, int kl, int ku, const 
         const int G = (nsteps + 63) >> 6;
         T w[NF];  // w[r]: block g + r
 #pragma unroll
-        for (int r = 0; r < NF; ++r) {
-            const int e = 64 * r + lane;
-            w[r] = e < n ? b[e] : T(0);
-        }
+        for (int r = 0; r < NF; ++r) w[r] = bload(64 * r + lane);
         T lp[PFD][NF];
         // multipliers of step j for window register r, lane = entry j + t
-        auto fetch = [&](int slot, int j, int s) {
-            const int lm = mini(kl, n - 1 - j);
-            const T *col = ab + (i64)j * ld + kd;
+        // Lanes outside rows 1..lm load whatever the band storage holds there
+        // (the update below is masked); only offsets off either end of AB
+        // matter, and those read 0.  A step past the end gets a base offset
+        // past the end of AB.
+        auto fetch = [&](int slot, int j, int s, int ln) {
+            const int base = j < nsteps ? (int)((i64)j * ld + kd) * SZ : kRingPastEnd;
 #pragma unroll
-            for (int r = 0; r < NF; ++r) {
-                const int t = 64 * r + lane - s;
-                lp[slot][r] = (j < nsteps && t >= 1 && t <= lm) ? col[t] : T(0);
-            }
+            for (int r = 0; r < NF; ++r) lp[slot][r] = ring_ld(rab, base + (64 * r + ln - s) * SZ, T());
         };
         Unroll<PFD>::run([&](auto sc) {
             constexpr int s = decltype(sc)::value;
-            fetch(s, s, s);
+            fetch(s, s, s, lane);
         });
-        int pvc = lane < nsteps ? (int)(ipiv[lane] - 1) : 0;
+        auto piv = [&](int e) { return (int)__builtin_amdgcn_raw_buffer_load_b32(rip, e < nsteps ? e * 8 : -1, 0, 0) - 1; };
+        int pvc = piv(lane);
         for (int g = 0; g < G; ++g) {
             const int j0 = 64 * g;
-            const int pvn = (j0 + 64 + lane < nsteps) ? (int)(ipiv[j0 + 64 + lane] - 1) : 0;
-            const int ein = 64 * (g + NF) + lane;
-            const T bnext = ein < n ? b[ein] : T(0);
+            // the lane index goes through an empty asm once per block, so the
+            // per-step lane offsets are recomputed (one add) instead of being
+            // hoisted out of the loop as 64 * NF live registers
+            int ln = lane;
+            asm volatile("" : "+v"(ln));
+            const int pvn = piv(j0 + 64 + lane);
+            const T bnext = bload(64 * (g + NF) + lane);
             Unroll<64>::run([&](auto sc) {
                 constexpr int s = decltype(sc)::value;
                 constexpr int slot = s % PFD;
@@ -103,7 +125,11 @@ __global__ void __launch_bounds__(64) gbtrs_n_ring(int n, int kl, int ku, const 
                     if (p != j) {
                         const int pos = s + (p - j);
                         const int rr = pos >> 6, pl = pos & 63;
-                        const T bp = lane_read(reg_pick(w, rr), pl);
+                        T wsel = w[0];
+#pragma unroll
+                        for (int r = 1; r < NF; ++r)
+                            if (rr == r) wsel = w[r];
+                        const T bp = lane_read(wsel, pl);
                         const T bj0 = lane_read(w[0], s);
                         w[0] = lane_write(w[0], bp, s);
 #pragma unroll
@@ -113,13 +139,14 @@ __global__ void __launch_bounds__(64) gbtrs_n_ring(int n, int kl, int ku, const 
                     const T nb = -lane_read(w[0], s);
 #pragma unroll
                     for (int r = 0; r < NF; ++r) {
-                        const int t = 64 * r + lane - s;
-                        if (t >= 1 && t <= lm) w[r] = dfma(lp[slot][r], nb, w[r]);
+                        const int t = 64 * r + ln - s;
+                        const T u = dfma(lp[slot][r], nb, w[r]);
+                        w[r] = (t >= 1 && t <= lm) ? u : w[r];
                     }
                 }
-                fetch(slot, j + PFD, (s + PFD) & 63);
+                fetch(slot, j + PFD, (s + PFD) & 63, ln);
             });
-            if (j0 + lane < n) b[j0 + lane] = w[0];
+            ring_st(rb, (j0 + lane < n) ? (j0 + lane) * SZ : -1, w[0]);
 #pragma unroll
             for (int r = 0; r + 1 < NF; ++r) w[r] = w[r + 1];
             w[NF - 1] = bnext;
@@ -128,38 +155,34 @@ __global__ void __launch_bounds__(64) gbtrs_n_ring(int n, int kl, int ku, const 
 #pragma unroll
         for (int r = 0; r < NF; ++r) {
             const int e = 64 * (G + r) + lane;
-            if (e < n) b[e] = w[r];
+            ring_st(rb, e < n ? e * SZ : -1, w[r]);
         }
     }
 
+    // the back substitution reads what the forward pass stored
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     // ------------------------------------------------------------ backward: U, kl+ku super-diagonals
     {
         const int G0 = (n - 1) >> 6;
         T w[NB];  // w[r]: block g - r
 #pragma unroll
-        for (int r = 0; r < NB; ++r) {
-            const int e = 64 * (G0 - r) + lane;
-            w[r] = (e >= 0 && e < n) ? b[e] : T(0);
-        }
+        for (int r = 0; r < NB; ++r) w[r] = bload(64 * (G0 - r) + lane);
         T up[PFD][NB];
         // column j of U for window register r, lane = entry j - t (t = 0: diagonal)
-        auto fetch = [&](int slot, int j, int s) {
-            const int tm = mini(kd, j);
-            const T *col = ab + (i64)j * ld + kd;
+        auto fetch = [&](int slot, int j, int s, int ln) {
+            const int base = (j >= 0 && j < n) ? (int)((i64)j * ld + kd) * SZ : kRingPastEnd;
 #pragma unroll
-            for (int r = 0; r < NB; ++r) {
-                const int t = s - lane + 64 * r;
-                up[slot][r] = (j >= 0 && j < n && t >= 0 && t <= tm) ? col[-t] : T(0);
-            }
+            for (int r = 0; r < NB; ++r) up[slot][r] = ring_ld(rab, base - (s - ln + 64 * r) * SZ, T());
         };
         Unroll<PFD>::run([&](auto sc) {
             constexpr int s = 63 - decltype(sc)::value;
-            fetch(s % PFD, 64 * G0 + s, s);
+            fetch(s % PFD, 64 * G0 + s, s, lane);
         });
         for (int g = G0; g >= 0; --g) {
             const int j0 = 64 * g;
-            const int ein = 64 * (g - NB) + lane;
-            const T bnext = ein >= 0 ? b[ein] : T(0);
+            int ln = lane;
+            asm volatile("" : "+v"(ln));
+            const T bnext = bload(64 * (g - NB) + lane);
             Unroll<64>::run([&](auto sc) {
                 constexpr int s = 63 - decltype(sc)::value;
                 constexpr int slot = s % PFD;
@@ -172,13 +195,14 @@ __global__ void __launch_bounds__(64) gbtrs_n_ring(int n, int kl, int ku, const 
                     const int tm = mini(kd, j);
 #pragma unroll
                     for (int r = 0; r < NB; ++r) {
-                        const int t = s - lane + 64 * r;
-                        if (t >= 1 && t <= tm) w[r] = dfma(up[slot][r], nx, w[r]);
+                        const int t = s - ln + 64 * r;
+                        const T u = dfma(up[slot][r], nx, w[r]);
+                        w[r] = (t >= 1 && t <= tm) ? u : w[r];
                     }
                 }
-                fetch(slot, j - PFD, (s - PFD) & 63);
+                fetch(slot, j - PFD, (s - PFD) & 63, ln);
             });
-            if (j0 + lane < n) b[j0 + lane] = w[0];
+            ring_st(rb, (j0 + lane < n) ? (j0 + lane) * SZ : -1, w[0]);
 #pragma unroll
             for (int r = 0; r + 1 < NB; ++r) w[r] = w[r + 1];
             w[NB - 1] = bnext;
@@ -193,7 +217,7 @@ template <typename T>
 int ring_gbtrs_launch(i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 ld, const i64 *ipiv, T *B, i64 ldb,
                       hipStream_t st) {
     const i64 kd = kl + ku;
-    if (n >= (1ll << 30) || kl > 128 || kd > 256) return -1;
+    if (n >= (1ll << 30) || kl > 128 || kd > 256 || ld * n * (i64)sizeof(T) >= (1ll << 31) - (1ll << 24)) return -1;
     const int nf = 1 + (int)cdiv(kl, 64), nb = 1 + (int)cdiv(kd, 64);
     constexpr int PFD = sizeof(T) == 4 ? 32 : 16;
     dim3 grid((unsigned)nrhs), block(64);

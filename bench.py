@@ -396,11 +396,13 @@ def main():
     lu_tf = lu_fl / (P["lu32_ms"] * 1e-3) / 1e12
     trs_fl = solve_flops(n, l, u, nrhs)
     trs_tf = trs_fl / (P["trs32_ms"] * 1e-3) / 1e12
-    traffic = None
+    traffic = traffic_gbmv = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("lu32_bytes_per_launch")
+            tj = json.load(f)
+        traffic = tj.get("lu32_bytes_per_launch")
+        traffic_gbmv = tj.get("gbmv64_bytes_per_launch")
     line = {
         "metric": "A\\b solves/s n=65536 l=u=128 mixed vs FP64 at 1e-12 err; gbmv HBM GB/s",
         "value": 1000.0 / P["ms_mixed"],
@@ -430,8 +432,10 @@ def main():
         "roofline_solve": {"kernel": "gbtrs_n_ring<float,3,5,32> (16 RHS, FP32)", "bound": "mfma", "achieved": trs_tf,
                            "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": trs_tf / FP32_PEAK_TFLOPS,
                            "kernel_ms": P["trs32_ms"], "flops_per_launch": trs_fl},
-        "roofline_gbmv": {"bound": "hbm", "achieved": g["GBps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                          "frac": g["GBps"] / HBM_PEAK_GBPS, "traffic": None},
+        "roofline_gbmv": {"kernel": "gbmv_n_wave<double,1,11,2> (FP64 gbmv 'N', n=2^20, (8,8))", "bound": "hbm",
+                          "achieved": g["GBps"], "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                          "frac": g["GBps"] / HBM_PEAK_GBPS, "traffic": traffic_gbmv,
+                          "bytes_per_launch": g["bytes"], "kernel_us": g["us"]},
     }
     line.update(out)
     line["workloads"] = extra
