@@ -1,0 +1,146 @@
+# BandedMatricesHIPExt -- routes BandedMatrices.jl's banded hot path for
+# device-resident band data (AMDGPU.ROCArray) to libbandedhip.so.
+#
+# Loaded by Julia's package-extension mechanism (Project.toml [weakdeps] /
+# [extensions], as the reference already does for its other extensions,
+# Project.toml:11-17) when both BandedMatrices and AMDGPU are present.
+# Every method below replaces one reference method for the device layout:
+#   mul!(y, A, x, α, β)   materialize!(::MulAdd{BandedColumns,...})  src/generic/matmul.jl:61-64
+#   A' * x                 BandedRows gbmv                            src/generic/matmul.jl:66-92
+#   mul!(Y, A, X, α, β)    dense multi-RHS                            src/generic/matmul.jl:243-256
+#   A * B  (banded)        gbmm!                                      src/banded/gbmm.jl:207-340
+#   lu / lu!               LAPACK.gbtrf!                              src/banded/BandedLU.jl:90-111
+#   ldiv!(F, B), F' \ B    LAPACK.gbtrs!                              src/banded/linalg.jl:24-47
+#   A \ b                  materialize!(::Ldiv{BandedColumns})        src/banded/linalg.jl:5-9,75
+#   qr / ldiv!(QR, B)      _banded_qr!, banded_qr_lmul!, tbsv!        src/banded/bandedqr.jl:1-214
+module BandedMatricesHIPExt
+
+using BandedMatrices, LinearAlgebra, ArrayLayouts, AMDGPU
+import BandedMatrices: BandedColumns, bandeddata, bandwidths, BandedLU, _BandedMatrix
+import ArrayLayouts: MemoryLayout, MulAdd, Ldiv, materialize!
+import LinearAlgebra: BlasInt
+
+const libbh = joinpath(@__DIR__, "..", "libbandedhip.so")
+
+# ---------------------------------------------------------------- layout hook
+# Device column-major layout; bandedcolumns(::HIPColumnMajor) makes a
+# BandedMatrix over a ROCMatrix dispatch to the methods below instead of the
+# scalar-indexing fallback (src/generic/matmul.jl:100-110).
+struct HIPColumnMajor <: ArrayLayouts.AbstractColumnMajor end
+MemoryLayout(::Type{<:ROCArray{T,N}}) where {T,N} = HIPColumnMajor()
+BandedMatrices.bandedcolumns(::HIPColumnMajor) = BandedColumns{HIPColumnMajor}()
+const HIPBanded{T} = BandedMatrix{T,<:ROCMatrix{T}}
+
+# ---------------------------------------------------------------- errors
+# Return codes (include/bandedhip.h): 0 ok; -k: argument k invalid
+# (ArgumentError, as LAPACK.chkargsok); BH_ERR_* <= -1000 runtime failures,
+# -1003 BandError.  A singular pivot column comes back in `info`, not here.
+const BH_ERR_BANDERROR = -1003
+function chk(r::Integer, name)
+    r == 0 && return nothing
+    r == BH_ERR_BANDERROR && throw(BandedMatrices.BandError())
+    r < 0 && r > -1000 && throw(ArgumentError("$name: invalid argument $(-r)"))
+    msg = unsafe_string(ccall((:bh_last_error, libbh), Cstring, ()))
+    error("$name: $msg")
+end
+
+dptr(a::ROCArray{T}) where {T} = reinterpret(Ptr{T}, pointer(a))
+
+# Device copies of the pivots, keyed by the host ipiv vector a BandedLU holds
+# (BandedLU.jl:12 stores ipiv on the host); gbtrs reads the device copy.
+const IPIV_DEV = WeakKeyDict{Vector{BlasInt},ROCVector{Int64}}()
+
+function __init__()
+    chk(ccall((:bh_init, libbh), Cint, (Cint,), AMDGPU.device_id(AMDGPU.device()) - 1), :bh_init)
+end
+
+# Library calls go on AMDGPU's current stream so they order with other device work.
+with_stream(f) = (ccall((:bh_set_stream, libbh), Cint, (Ptr{Cvoid},), AMDGPU.stream().stream); f())
+
+for (T, p) in ((Float64, :d), (Float32, :s))
+    gbmv = Symbol(:bh_, p, :gbmv); gbmv_multi = Symbol(:bh_, p, :gbmv_multi); gbmm = Symbol(:bh_, p, :gbmm)
+    gbtrf = Symbol(:bh_, p, :gbtrf); gbtrs = Symbol(:bh_, p, :gbtrs); gbpad = Symbol(:bh_, p, :gbpad)
+    gbqrf = Symbol(:bh_, p, :gbqrf); gbqrs = Symbol(:bh_, p, :gbqrs)
+    @eval begin
+        # y <- α A x + β y   (BLAS.gbmv! in banded_gbmv!, src/generic/matmul.jl:21-23)
+        function _gbmv!(trans::Char, α::$T, A::HIPBanded{$T}, x::ROCVector{$T}, β::$T, y::ROCVector{$T})
+            l, u = bandwidths(A); d = bandeddata(A); m, n = size(A)
+            with_stream() do
+                chk(ccall(($(QuoteNode(gbmv)), libbh), Cint,
+                          (UInt8, Int64, Int64, Int64, Int64, $T, Ptr{$T}, Int64, Ptr{$T}, Int64, $T, Ptr{$T}, Int64),
+                          trans, m, n, l, u, α, dptr(d), stride(d, 2), dptr(x), 1, β, dptr(y), 1), $(QuoteNode(gbmv)))
+            end
+            y
+        end
+        function _gbmv_multi!(trans::Char, α::$T, A::HIPBanded{$T}, X::ROCMatrix{$T}, β::$T, Y::ROCMatrix{$T})
+            l, u = bandwidths(A); d = bandeddata(A); m, n = size(A)
+            with_stream() do
+                chk(ccall(($(QuoteNode(gbmv_multi)), libbh), Cint,
+                          (UInt8, Int64, Int64, Int64, Int64, Int64, $T, Ptr{$T}, Int64, Ptr{$T}, Int64, $T, Ptr{$T}, Int64),
+                          trans, m, n, l, u, size(X, 2), α, dptr(d), stride(d, 2), dptr(X), stride(X, 2), β,
+                          dptr(Y), stride(Y, 2)), $(QuoteNode(gbmv_multi)))
+            end
+            Y
+        end
+        # C <- α A B + β C   (gbmm!, src/banded/gbmm.jl:207-340)
+        function _gbmm!(α::$T, A::HIPBanded{$T}, B::HIPBanded{$T}, β::$T, C::HIPBanded{$T})
+            (Al, Au), (Bl, Bu), (Cl, Cu) = bandwidths(A), bandwidths(B), bandwidths(C)
+            a, b, c = bandeddata(A), bandeddata(B), bandeddata(C)
+            with_stream() do
+                chk(ccall(($(QuoteNode(gbmm)), libbh), Cint,
+                          (Int64, Int64, Int64, $T, Ptr{$T}, Int64, Int64, Int64, Ptr{$T}, Int64, Int64, Int64,
+                           $T, Ptr{$T}, Int64, Int64, Int64),
+                          size(A, 1), size(A, 2), size(B, 2), α, dptr(a), stride(a, 2), Al, Au, dptr(b), stride(b, 2),
+                          Bl, Bu, β, dptr(c), stride(c, 2), Cl, Cu), $(QuoteNode(gbmm)))
+            end
+            C
+        end
+        # lu!: padded copy (BandedMatrix{T}(A,(l,l+u)), BandedLU.jl:110) then gbtrf
+        # (BandedLU.jl:98).  ipiv is returned on the host, as BandedLU stores it.
+        function _lu(A::HIPBanded{$T})
+            l, u = bandwidths(A); m, n = size(A); d = bandeddata(A)
+            AB = ROCMatrix{$T}(undef, 2l + u + 1, n)
+            ipiv_d = ROCVector{Int64}(undef, min(m, n)); info = Ref{Int64}(0)
+            with_stream() do
+                chk(ccall(($(QuoteNode(gbpad)), libbh), Cint, (Int64, Int64, Int64, Int64, Ptr{$T}, Int64, Ptr{$T}, Int64),
+                          m, n, l, u, dptr(d), stride(d, 2), dptr(AB), stride(AB, 2)), $(QuoteNode(gbpad)))
+                chk(ccall(($(QuoteNode(gbtrf)), libbh), Cint,
+                          (Int64, Int64, Int64, Int64, Ptr{$T}, Int64, Ptr{Int64}, Ref{Int64}),
+                          m, n, l, u, dptr(AB), stride(AB, 2), dptr(ipiv_d), info), $(QuoteNode(gbtrf)))
+            end
+            F = BandedLU{$T}(_BandedMatrix(AB, m, l, l + u), Vector{BlasInt}(Array(ipiv_d)), BlasInt(info[]))
+            IPIV_DEV[F.ipiv] = ipiv_d      # device mirror for gbtrs
+            F
+        end
+        # ldiv!(F, B) / transpose(F) \ B   (LAPACK.gbtrs!, linalg.jl:24-47)
+        function _gbtrs!(trans::Char, F::BandedLU{$T,<:HIPBanded{$T}}, B::ROCVecOrMat{$T})
+            AB = F.factors; n = size(AB, 2); l, uu = bandwidths(AB); u = uu - l; d = bandeddata(AB)
+            ipiv_d = get!(() -> ROCVector{Int64}(F.ipiv), IPIV_DEV, F.ipiv)
+            ldb = B isa ROCVector ? length(B) : stride(B, 2)
+            with_stream() do
+                chk(ccall(($(QuoteNode(gbtrs)), libbh), Cint,
+                          (UInt8, Int64, Int64, Int64, Int64, Ptr{$T}, Int64, Ptr{Int64}, Ptr{$T}, Int64),
+                          trans, n, l, u, size(B, 2), dptr(d), stride(d, 2), dptr(ipiv_d), dptr(B), ldb),
+                    $(QuoteNode(gbtrs)))
+            end
+            B
+        end
+    end
+end
+
+# ---------------------------------------------------------------- dispatch
+const HIPBC = BandedColumns{HIPColumnMajor}
+materialize!(M::MulAdd{HIPBC,HIPColumnMajor,HIPColumnMajor,<:Any,<:HIPBanded,<:ROCVector,<:ROCVector}) =
+    _gbmv!('N', M.α, M.A, M.B, M.β, M.C)
+materialize!(M::MulAdd{HIPBC,HIPColumnMajor,HIPColumnMajor,<:Any,<:HIPBanded,<:ROCMatrix,<:ROCMatrix}) =
+    _gbmv_multi!('N', M.α, M.A, M.B, M.β, M.C)
+materialize!(M::MulAdd{HIPBC,HIPBC,HIPBC}) = _gbmm!(M.α, M.A, M.B, M.β, M.C)
+LinearAlgebra.mul!(y::ROCVector{T}, A::Adjoint{T,<:HIPBanded{T}}, x::ROCVector{T}, α::Number, β::Number) where {T<:Union{Float32,Float64}} =
+    _gbmv!('T', T(α), parent(A), x, T(β), y)
+ArrayLayouts._lu(::HIPBC, axes, A::HIPBanded; kwds...) = _lu(A)
+LinearAlgebra.ldiv!(F::BandedLU{T,<:HIPBanded{T}}, B::ROCVecOrMat{T}) where {T} = _gbtrs!('N', F, B)
+LinearAlgebra.ldiv!(F::Transpose{T,<:BandedLU{T,<:HIPBanded{T}}}, B::ROCVecOrMat{T}) where {T} = _gbtrs!('T', parent(F), B)
+LinearAlgebra.ldiv!(F::Adjoint{T,<:BandedLU{T,<:HIPBanded{T}}}, B::ROCVecOrMat{T}) where {T<:Real} = _gbtrs!('T', parent(F), B)
+materialize!(M::Ldiv{HIPBC}) = ldiv!(lu(M.A), M.B)
+
+end # module
