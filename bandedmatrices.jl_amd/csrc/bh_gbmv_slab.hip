@@ -20,6 +20,26 @@ namespace bh {
 
 namespace {
 
+template <int K>
+struct CIdx {
+    static constexpr int value = K;
+};
+// f(CIdx<0>{}) ... f(CIdx<N-1>{}): loop indices that are compile-time
+// constants from the first pass on, so register arrays stay in registers.
+template <int N>
+struct CFor {
+    template <class F>
+    __device__ __forceinline__ static void run(F &&f) {
+        CFor<N - 1>::run(f);
+        f(CIdx<N - 1>{});
+    }
+};
+template <>
+struct CFor<0> {
+    template <class F>
+    __device__ __forceinline__ static void run(F &&) {}
+};
+
 // Block b of a grid of nb runs on XCD b mod 8; give each XCD one contiguous
 // range of tiles so that the halo columns two neighbouring tiles share are
 // read through the same L2.
@@ -74,12 +94,23 @@ __global__ void __launch_bounds__(NT) gbmv_n_slab(int m, int n, int kl, int ku, 
         pre[q] = v;
     }
     constexpr int XPT = RPT + 1;  // x values per thread (nc <= R + kl + ku <= XPT * NT when kl+ku <= NT)
-    T xv[XPT];
-#pragma unroll
-    for (int q = 0; q < XPT; ++q) {
-        const int c = tid + q * NT;
-        xv[q] = (c < nc) ? X[c0 + c] : T(0);
-    }
+    constexpr int XD = 8;         // right-hand sides whose x is in flight
+    // x of right-hand sides r .. r+XD-1 are loaded into registers XD
+    // right-hand sides ahead of their use, so with many right-hand sides the
+    // x loads overlap the products instead of waiting once per right-hand side
+    T xq[XD][XPT];
+    auto load_x = [&](T(&xv)[XPT], int r) __attribute__((always_inline)) {
+        const T *x = X + (i64)r * ldx + c0;
+        CFor<XPT>::run([&](auto qi) {
+            constexpr int q = decltype(qi)::value;
+            const int c = tid + q * NT;
+            xv[q] = (r < nrhs && c < nc) ? x[c] : T(0);
+        });
+    };
+    CFor<XD>::run([&](auto di) {
+        constexpr int d = decltype(di)::value;
+        load_x(xq[d], d);
+    });
 #pragma unroll
     for (int q = 0; q < NPF; ++q) {
         const int k = tid + q * NT;
@@ -88,65 +119,65 @@ __global__ void __launch_bounds__(NT) gbmv_n_slab(int m, int n, int kl, int ku, 
     const T *slab = reinterpret_cast<const T *>(lds + shift);  // slab[c*h + t]: band row t of column c0+c
     T *ax = reinterpret_cast<T *>(lds + ax_off);
 
-    for (int r = 0; r < nrhs; ++r) {
-        if (r == 0) {
-#pragma unroll
-            for (int q = 0; q < XPT; ++q) {
+    for (int rb = 0; rb < nrhs; rb += XD) {
+        CFor<XD>::run([&](auto di) {
+            constexpr int d = decltype(di)::value;
+            const int r = rb + d;
+            if (r >= nrhs) return;  // uniform
+            CFor<XPT>::run([&](auto qi) {
+                constexpr int q = decltype(qi)::value;
                 const int c = tid + q * NT;
-                if (c < nc) ax[c] = alpha * xv[q];
-            }
-        } else {
-            const T *x = X + (i64)r * ldx;
-            for (int c = tid; c < nc; c += NT) ax[c] = alpha * x[c0 + c];
-        }
-        __syncthreads();
-        if (interior) {
-            // every row has all h band entries inside the matrix: a uniform
-            // trip count, so the LDS reads of eight steps are issued before
-            // their fmas consume them (same ascending-j fma order)
+                if (c < nc) ax[c] = alpha * xq[d][q];
+            });
+            __syncthreads();
+            load_x(xq[d], r + XD);
+            if (interior) {
+                // every row has all h band entries inside the matrix: a uniform
+                // trip count, so the LDS reads of eight steps are issued before
+                // their fmas consume them (same ascending-j fma order)
 #pragma unroll
-            for (int q = 0; q < RPT; ++q) {
-                const int li = tid + q * NT;
-                T *yp = Y + (i64)r * ldy + r0 + li;
-                T acc = (beta == T(0)) ? T(0) : beta * *yp;
-                const T *ap = slab + li * h + (kl + ku);  // entry (i, i-kl+d) at ap[d*(h-1)]
-                const T *xp = ax + li;                    // alpha*x_{i-kl+d} at xp[d]
-                int d = 0;
-                for (; d + 8 <= h; d += 8) {
-                    T av[8], xw[8];
+                for (int q = 0; q < RPT; ++q) {
+                    const int li = tid + q * NT;
+                    T *yp = Y + (i64)r * ldy + r0 + li;
+                    T acc = (beta == T(0)) ? T(0) : beta * *yp;
+                    const T *ap = slab + li * h + (kl + ku);  // entry (i, i-kl+d) at ap[d*(h-1)]
+                    const T *xp = ax + li;                    // alpha*x_{i-kl+d} at xp[d]
+                    int e = 0;
+                    for (; e + 8 <= h; e += 8) {
+                        T av[8], xw[8];
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        av[u] = ap[(d + u) * (h - 1)];
-                        xw[u] = xp[d + u];
+                        for (int u = 0; u < 8; ++u) {
+                            av[u] = ap[(e + u) * (h - 1)];
+                            xw[u] = xp[e + u];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) acc = dfma(xw[u], av[u], acc);
                     }
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) acc = dfma(xw[u], av[u], acc);
+                    for (; e < h; ++e) acc = dfma(xp[e], ap[e * (h - 1)], acc);
+                    *yp = acc;
                 }
-                for (; d < h; ++d) acc = dfma(xp[d], ap[d * (h - 1)], acc);
-                *yp = acc;
+            } else {
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) {
+                    const int i = r0 + tid + q * NT;
+                    if (i < m) {
+                        T *yp = Y + (i64)r * ldy + i;
+                        T acc = (beta == T(0)) ? T(0) : beta * *yp;
+                        const int jlo = i - kl > 0 ? i - kl : 0;
+                        const int jhi = (i + ku < n - 1) ? i + ku : n - 1;
+                        const T *ap = slab + (jlo - c0) * h + (ku + i - jlo);
+                        const T *xp = ax + (jlo - c0);
+                        for (int j = jlo; j <= jhi; ++j) {
+                            acc = dfma(*xp, *ap, acc);
+                            ++xp;
+                            ap += h - 1;
+                        }
+                        *yp = acc;
+                    }
+                }
             }
             __syncthreads();
-            continue;
-        }
-#pragma unroll
-        for (int q = 0; q < RPT; ++q) {
-            const int i = r0 + tid + q * NT;
-            if (i < m) {
-                T *yp = Y + (i64)r * ldy + i;
-                T acc = (beta == T(0)) ? T(0) : beta * *yp;
-                const int jlo = i - kl > 0 ? i - kl : 0;
-                const int jhi = (i + ku < n - 1) ? i + ku : n - 1;
-                const T *ap = slab + (jlo - c0) * h + (ku + i - jlo);
-                const T *xp = ax + (jlo - c0);
-                for (int j = jlo; j <= jhi; ++j) {
-                    acc = dfma(*xp, *ap, acc);
-                    ++xp;
-                    ap += h - 1;
-                }
-                *yp = acc;
-            }
-        }
-        __syncthreads();
+        });
     }
 }
 
@@ -194,26 +225,6 @@ int slab_launch(int m, int n, int kl, int ku, int nrhs, T alpha, const T *A, i64
 }
 
 typedef unsigned v4u __attribute__((ext_vector_type(4)));
-
-template <int K>
-struct CIdx {
-    static constexpr int value = K;
-};
-// f(CIdx<0>{}) ... f(CIdx<N-1>{}): loop indices that are compile-time
-// constants from the first pass on, so register arrays stay in registers.
-template <int N>
-struct CFor {
-    template <class F>
-    __device__ __forceinline__ static void run(F &&f) {
-        CFor<N - 1>::run(f);
-        f(CIdx<N - 1>{});
-    }
-};
-template <>
-struct CFor<0> {
-    template <class F>
-    __device__ __forceinline__ static void run(F &&) {}
-};
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
