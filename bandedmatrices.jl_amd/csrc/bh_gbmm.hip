@@ -46,19 +46,37 @@ __global__ void __launch_bounds__(kMmThreads) gbmm_kernel(GbmmArgs<T> p, i64 NC)
     const i64 v0 = imax(0, j0 - p.Bu), v1 = imin(p.k, j1 + p.Bl);
     T *sB = reinterpret_cast<T *>(smem_raw);             // NC columns of B
     T *sA = sB + NC * SB;                                // A columns [v0, v1)
-    for (i64 e = threadIdx.x; e < (j1 - j0) * hB; e += blockDim.x) {
-        i64 c = e / hB, t = e - c * hB;
-        sB[c * SB + t] = p.alpha * p.B[(j0 + c) * p.ldb + t];  // alpha*b, rounded as the reference's gbmv
+    // Staging: when the band storage has no padding (ld == h) and the LDS
+    // stride equals h, the columns are one contiguous run on both sides and
+    // are copied flat; otherwise element (column, row) indices come from a
+    // 32-bit division (the slab sizes are small).
+    const int nB = (int)((j1 - j0) * hB), nA = (int)((v1 - v0) * hA);
+    const int hBi = (int)hB, hAi = (int)hA, SBi = (int)SB, SAi = (int)SA;
+    if (p.ldb == hB && SB == hB) {
+        const T *src = p.B + j0 * p.ldb;
+        for (int e = threadIdx.x; e < nB; e += blockDim.x) sB[e] = p.alpha * src[e];  // alpha*b, as the reference's gbmv
+    } else {
+        for (int e = threadIdx.x; e < nB; e += blockDim.x) {
+            const int c = e / hBi, t = e - c * hBi;
+            sB[c * SBi + t] = p.alpha * p.B[(j0 + c) * p.ldb + t];
+        }
     }
-    for (i64 e = threadIdx.x; e < (v1 - v0) * hA; e += blockDim.x) {
-        i64 c = e / hA, t = e - c * hA;
-        sA[c * SA + t] = p.A[(v0 + c) * p.lda + t];
+    if (p.lda == hA && SA == hA) {
+        const T *src = p.A + v0 * p.lda;
+        for (int e = threadIdx.x; e < nA; e += blockDim.x) sA[e] = src[e];
+    } else {
+        for (int e = threadIdx.x; e < nA; e += blockDim.x) {
+            const int c = e / hAi, t = e - c * hAi;
+            sA[c * SAi + t] = p.A[(v0 + c) * p.lda + t];
+        }
     }
     __syncthreads();
 
     const i64 shift = p.Ctu - p.Cu;  // first stored row of the product bands
-    for (i64 e = threadIdx.x; e < (j1 - j0) * hC; e += blockDim.x) {
-        const i64 jj = e / hC, r = e - jj * hC, j = j0 + jj;
+    const int hCi = (int)hC, nC = (int)((j1 - j0) * hC);
+    for (int e = threadIdx.x; e < nC; e += blockDim.x) {
+        const int jj = e / hCi;
+        const i64 r = e - jj * hCi, j = j0 + jj;
         T *cp = p.C + j * p.ldc + r;
         const bool extra = (r < shift) || (r > p.Ctu + p.Cl);
         if (extra) {
@@ -73,8 +91,26 @@ __global__ void __launch_bounds__(kMmThreads) gbmm_kernel(GbmmArgs<T> p, i64 NC)
         if (i < 0 || i >= p.n) continue;  // outside the matrix: never touched
         T acc = (p.beta == T(0)) ? T(0) : p.beta * *cp;
         const i64 vlo = imax(imax(0, j - p.Bu), i - p.Al), vhi = imin(imin(p.k - 1, j + p.Bl), i + p.Au);
-        for (i64 v = vlo; v <= vhi; ++v)
-            acc = dfma(sB[jj * SB + (p.Bu + v - j)], sA[(v - v0) * SA + (p.Au + i - v)], acc);
+        // v ascending: b_vj at sB[jj*SB + (Bu + v - j)], a_iv at sA[(v - v0)*SA + (Au + i - v)]
+        const T *bp = sB + jj * SBi + (int)(p.Bu + vlo - j);
+        const T *ap = sA + (int)(vlo - v0) * SAi + (int)(p.Au + i - vlo);
+        const int cnt = (int)(vhi - vlo) + 1, sa1 = SAi - 1;
+        int v = 0;
+        for (; v + 4 <= cnt; v += 4) {  // four steps' LDS reads issued before their fmas
+            const T b0 = bp[0], b1 = bp[1], b2 = bp[2], b3 = bp[3];
+            const T a0 = ap[0], a1 = ap[sa1], a2 = ap[2 * sa1], a3 = ap[3 * sa1];
+            acc = dfma(b0, a0, acc);
+            acc = dfma(b1, a1, acc);
+            acc = dfma(b2, a2, acc);
+            acc = dfma(b3, a3, acc);
+            bp += 4;
+            ap += 4 * sa1;
+        }
+        for (; v < cnt; ++v) {
+            acc = dfma(*bp, *ap, acc);
+            ++bp;
+            ap += sa1;
+        }
         *cp = acc;
     }
 }
