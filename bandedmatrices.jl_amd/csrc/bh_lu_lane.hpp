@@ -109,10 +109,19 @@ __device__ __forceinline__ unsigned long long abs_bits(double x) {
 }
 __device__ __forceinline__ unsigned abs_bits(float x) { return __builtin_bit_cast(unsigned, x) & 0x7fffffffu; }
 __device__ __forceinline__ unsigned long long inf_bits(double) { return 0x7ff0000000000000ull; }
+// value of lane l in every lane, into a vector register (ds_bpermute)
+__device__ __forceinline__ float lane_bcast(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(l << 2, __builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double lane_bcast(double v, int l) {
+    const long long x = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_ds_bpermute(l << 2, (int)x), hi = __builtin_amdgcn_ds_bpermute(l << 2, (int)(x >> 32));
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
 __device__ __forceinline__ unsigned inf_bits(float) { return 0x7f800000u; }
 
 template <typename T, int KL, bool SOLVE, int UMIN>
-__device__ int lane_gbsv_wave(int m, int n, int ku, T *__restrict__ ab, i64 ld, i64 *__restrict__ ipiv,
+__device__ __forceinline__ int lane_gbsv_wave(int m, int n, int ku, T *__restrict__ ab, i64 ld, i64 *__restrict__ ipiv,
                               T *__restrict__ b) {
     constexpr int R = KL + 1;
     constexpr int U = R * ((UMIN - 1 + R) / R);  // steps per unrolled block (>= UMIN, a multiple of R)
@@ -171,38 +180,66 @@ __device__ int lane_gbsv_wave(int m, int n, int ku, T *__restrict__ ab, i64 ld, 
         const int km = TAIL ? smin(KL, m - 1 - j) : KL;
         auto reg = [](int i) { return (i + s) % R; };
         // pivot: max |a| of the lane's column rows 0..km, then the first row of
-        // column j (broadcast) whose |a| equals column j's max
+        // column j whose |a| equals column j's max
         T mx = dabs(y[reg(0)]);
 #pragma unroll
         for (int i = 1; i < R; ++i)
             if (!TAIL || i <= km) mx = __builtin_fmax(mx, dabs(y[reg(i)]));
-        T xs[R];
-#pragma unroll
-        for (int i = 0; i < R; ++i) xs[i] = lane_read(y[reg(i)], own);
-        const auto mb = abs_bits(lane_read(mx, own));
         int jp = 0;
+        T pv;
+        T l[R];
+        bool nz;
+        if constexpr (R <= 9) {
+            // narrow windows: column j broadcast to scalars, scalar compares
+            T xs[R];
 #pragma unroll
-        for (int i = R - 1; i >= 1; --i)
-            if ((!TAIL || i <= km) && abs_bits(xs[i]) == mb) jp = i;
-        // row 0 also when its |a| is NaN: nothing compares greater than it
-        if (abs_bits(xs[0]) == mb || abs_bits(xs[0]) > inf_bits(T())) jp = 0;
-        T pv = xs[0];
+            for (int i = 0; i < R; ++i) xs[i] = lane_read(y[reg(i)], own);
+            const auto mb = abs_bits(lane_read(mx, own));
 #pragma unroll
-        for (int i = 1; i < R; ++i)
-            if (jp == i) pv = xs[i];
-        const bool nz = abs_bits(pv) != 0;  // NaN counts as nonzero, as in LAPACK's test
+            for (int i = R - 1; i >= 1; --i)
+                if ((!TAIL || i <= km) && abs_bits(xs[i]) == mb) jp = i;
+            // row 0 also when its |a| is NaN: nothing compares greater than it
+            if (abs_bits(xs[0]) == mb || abs_bits(xs[0]) > inf_bits(T())) jp = 0;
+            pv = xs[0];
+#pragma unroll
+            for (int i = 1; i < R; ++i)
+                if (jp == i) pv = xs[i];
+            nz = abs_bits(pv) != 0;  // NaN counts as nonzero, as in LAPACK's test
+            const T rcp = T(1) / pv;
+            // multipliers of the interchanged column: a * (1/pivot)
+#pragma unroll
+            for (int i = 1; i < R; ++i) {
+                const T xp = (jp == i) ? xs[0] : xs[i];
+                l[i] = nz ? xp * rcp : xp;
+            }
+        } else {
+            // wide windows (too many values for scalar registers): every lane
+            // finds the pivot row of its own column and forms its own
+            // multipliers; lane own's are broadcast into vector registers
+            const auto mbl = abs_bits(mx);
+            int jpl = 0;
+#pragma unroll
+            for (int i = R - 1; i >= 1; --i)
+                if ((!TAIL || i <= km) && abs_bits(y[reg(i)]) == mbl) jpl = i;
+            if (abs_bits(y[reg(0)]) == mbl || abs_bits(y[reg(0)]) > inf_bits(T())) jpl = 0;
+            jp = __builtin_amdgcn_readlane(jpl, own);
+            T ysel = y[reg(0)];
+#pragma unroll
+            for (int i = 1; i < R; ++i)
+                if (jp == i) ysel = y[reg(i)];
+            pv = lane_read(ysel, own);
+            nz = abs_bits(pv) != 0;
+            const T rcp = T(1) / pv;
+#pragma unroll
+            for (int i = 1; i < R; ++i) {
+                const T xp = (jp == i) ? y[reg(0)] : y[reg(i)];
+                l[i] = lane_bcast(nz ? xp * rcp : xp, own);
+            }
+        }
         if (nz)
             ju = smax(ju, smin(j + jp + ku, n - 1));
         else if (info == 0)
             info = j + 1;
-        // multipliers of the interchanged column: a * (1/pivot)
-        const T rcp = T(1) / pv;
-        T l[R];
-#pragma unroll
-        for (int i = 1; i < R; ++i) {
-            const T xp = (jp == i) ? xs[0] : xs[i];
-            l[i] = nz ? xp * rcp : xp;
-        }
         // interchange rows 0 and jp over columns j..ju only: a lane outside
         // them keeps its rows in place, so the rows a retired column leaves
         // behind are overwritten by incoming rows within R steps
@@ -310,6 +347,13 @@ __global__ void __launch_bounds__(64) gbtrf_lane_kernel(int m, int n, int ku, T 
 
 // One body per subdiagonal count (inlined; the kernel's register budget is
 // that of the widest).
+// One outlined body per subdiagonal count, so the batched kernel is a switch
+// over calls rather than nine inlined copies.
+template <typename T, int KL>
+__device__ __noinline__ int batch_sys(int n, int ku, T *ab, i64 ld, i64 *ip, T *b) {
+    return lane_gbsv_wave<T, KL, true, 8>(n, n, ku, ab, ld, ip, b);
+}
+
 template <typename T>
 __device__ __forceinline__ int batch_one(const BatchDesc &d, T *AB, T *Bv, i64 *IP) {
     const i64 ld = 2 * d.kl + d.ku + 1;
@@ -319,7 +363,7 @@ __device__ __forceinline__ int batch_one(const BatchDesc &d, T *AB, T *Bv, i64 *
     switch (d.kl) {
 #define BH_BATCH_CASE(KLv) \
     case KLv:              \
-        return lane_gbsv_wave<T, KLv, true, 8>(d.n, d.n, d.ku, ab, ld, ip, b);
+        return batch_sys<T, KLv>(d.n, d.ku, ab, ld, ip, b);
         BH_BATCH_CASE(0)
         BH_BATCH_CASE(1)
         BH_BATCH_CASE(2)
@@ -329,7 +373,7 @@ __device__ __forceinline__ int batch_one(const BatchDesc &d, T *AB, T *Bv, i64 *
         BH_BATCH_CASE(6)
         BH_BATCH_CASE(7)
         default:
-            return lane_gbsv_wave<T, 8, true, 8>(d.n, d.n, d.ku, ab, ld, ip, b);
+            return batch_sys<T, 8>(d.n, d.ku, ab, ld, ip, b);
 #undef BH_BATCH_CASE
     }
 }
