@@ -189,7 +189,7 @@ __device__ __forceinline__ int lane_gbsv_wave(int m, int n, int ku, T *__restric
         T pv;
         T l[R];
         bool nz;
-        if constexpr (R <= 5) {
+        if constexpr (R <= 3) {
             // narrow windows: column j broadcast to scalars, scalar compares
             T xs[R];
 #pragma unroll
