@@ -210,6 +210,149 @@ __global__ void __launch_bounds__(64) gbtrs_n_ring(int n, int kl, int ku, const 
     }
 }
 
+// 'T' solve (transpose(F) \ B, src/banded/linalg.jl:41-47): U' y = b with
+// j ascending, x_j = (b_j - sum_t u_{j-t,j} x_{j-t}) / u_jj, then L' with the
+// interchanges in reverse, b_j -= sum_t l_{j+t,j} b_{j+t} followed by the swap
+// b_j <-> b_{ipiv_j}.  Each step is a dot product over the band entries of
+// one factor column (contiguous in band storage, so one coalesced load per
+// window register) with the finished entries held in the lane ring, summed
+// across the wave with DPP.  The dots are summed in a different order than
+// the CPU routine's, so results agree to rounding, not bit for bit.
+template <typename T, int NF, int NB, int PFD>
+__global__ void __launch_bounds__(64) gbtrs_t_ring(int n, int kl, int ku, const T *__restrict__ ab, i64 ld,
+                                                   const i64 *__restrict__ ipiv, T *B, i64 ldb) {
+    static_assert(64 % PFD == 0, "prefetch distance divides the block");
+    constexpr int SZ = (int)sizeof(T);
+    const int lane = threadIdx.x;
+    T *b = B + (i64)blockIdx.x * ldb;
+    const int kd = kl + ku;
+    const __amdgpu_buffer_rsrc_t rab = ring_rsrc(ab, ld * (i64)n * SZ);
+    const __amdgpu_buffer_rsrc_t rb = ring_rsrc(b, (i64)n * SZ);
+    const __amdgpu_buffer_rsrc_t rip = ring_rsrc(ipiv, (i64)n * 8);
+    auto bload = [&](int e) { return ring_ld(rb, (e >= 0 && e < n) ? e * SZ : -1, T()); };
+
+    // ------------------------------------------------------------ U' (ascending)
+    {
+        const int G = (n + 63) >> 6;
+        T P[NB];  // P[r]: block g - r; entries below j are finished
+#pragma unroll
+        for (int r = 0; r < NB; ++r) P[r] = bload(64 * (0 - r) + lane);
+        T up[PFD][NB];
+        // column j of U: lane L of register r holds u_{64(g-r)+L, j}, at t = s - L + 64r above the diagonal
+        auto fetch = [&](int slot, int j, int s, int ln) {
+            const int base = j < n ? (int)((i64)j * ld + kd) * SZ : kRingPastEnd;
+#pragma unroll
+            for (int r = 0; r < NB; ++r) up[slot][r] = ring_ld(rab, base - (s - ln + 64 * r) * SZ, T());
+        };
+        Unroll<PFD>::run([&](auto sc) {
+            constexpr int s = decltype(sc)::value;
+            fetch(s, s, s, lane);
+        });
+        for (int g = 0; g < G; ++g) {
+            const int j0 = 64 * g;
+            int ln = lane;
+            asm volatile("" : "+v"(ln));
+            const T bnext = bload(64 * (g + 1) + lane);
+            Unroll<64>::run([&](auto sc) {
+                constexpr int s = decltype(sc)::value;
+                constexpr int slot = s % PFD;
+                const int j = j0 + s;
+                if (j < n) {
+                    const int tm = mini(kd, j);
+                    T part = T(0);
+#pragma unroll
+                    for (int r = 0; r < NB; ++r) {
+                        const int t = s - ln + 64 * r;
+                        const T q = dfma(up[slot][r], P[r], part);
+                        part = (t >= 1 && t <= tm) ? q : part;
+                    }
+                    const T dot = wave_sum_dpp(part);
+                    const T xj = (lane_read(P[0], s) - dot) / lane_read(up[slot][0], s);
+                    P[0] = lane_write(P[0], xj, s);
+                }
+                fetch(slot, j + PFD, (s + PFD) & 63, ln);
+            });
+            ring_st(rb, (j0 + lane < n) ? (j0 + lane) * SZ : -1, P[0]);
+#pragma unroll
+            for (int r = NB - 1; r >= 1; --r) P[r] = P[r - 1];
+            P[0] = bnext;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    // ------------------------------------------------------------ L' and the interchanges (descending)
+    if (kl > 0 && n > 1) {
+        const int g0 = (n - 2) >> 6;
+        T F[NF];  // F[r]: block g + r; entries above j are finished
+#pragma unroll
+        for (int r = 0; r < NF; ++r) F[r] = bload(64 * (g0 + r) + lane);
+        T lp[PFD][NF];
+        // column j of L: lane L of register r holds l_{64(g+r)+L, j}, at t = 64r + L - s below the diagonal
+        auto fetch = [&](int slot, int j, int s, int ln) {
+            const int base = (j >= 0 && j < n - 1) ? (int)((i64)j * ld + kd) * SZ : kRingPastEnd;
+#pragma unroll
+            for (int r = 0; r < NF; ++r) lp[slot][r] = ring_ld(rab, base + (64 * r + ln - s) * SZ, T());
+        };
+        Unroll<PFD>::run([&](auto sc) {
+            constexpr int s = 63 - decltype(sc)::value;
+            fetch(s % PFD, 64 * g0 + s, s, lane);
+        });
+        auto piv = [&](int e) {
+            return (int)__builtin_amdgcn_raw_buffer_load_b32(rip, (e >= 0 && e < n - 1) ? e * 8 : -1, 0, 0) - 1;
+        };
+        for (int g = g0; g >= 0; --g) {
+            const int j0 = 64 * g;
+            int ln = lane;
+            asm volatile("" : "+v"(ln));
+            const int pvb = piv(j0 + lane);
+            const T bprev = bload(64 * (g - 1) + lane);
+            Unroll<64>::run([&](auto sc) {
+                constexpr int s = 63 - decltype(sc)::value;
+                constexpr int slot = s % PFD;
+                const int j = j0 + s;
+                if (j <= n - 2) {
+                    const int lm = mini(kl, n - 1 - j);
+                    T part = T(0);
+#pragma unroll
+                    for (int r = 0; r < NF; ++r) {
+                        const int t = 64 * r + ln - s;
+                        const T q = dfma(lp[slot][r], F[r], part);
+                        part = (t >= 1 && t <= lm) ? q : part;
+                    }
+                    const T dot = wave_sum_dpp(part);
+                    T bj = lane_read(F[0], s) - dot;
+                    const int p = __builtin_amdgcn_readlane(pvb, s);
+                    if (p != j) {
+                        const int pos = s + (p - j);
+                        const int rr = pos >> 6, pl = pos & 63;
+                        T fsel = F[0];
+#pragma unroll
+                        for (int r = 1; r < NF; ++r)
+                            if (rr == r) fsel = F[r];
+                        const T bp = lane_read(fsel, pl);
+#pragma unroll
+                        for (int r = 0; r < NF; ++r)
+                            if (r == rr) F[r] = lane_write(F[r], bj, pl);
+                        bj = bp;
+                    }
+                    F[0] = lane_write(F[0], bj, s);
+                }
+                fetch(slot, j - PFD, (s - PFD) & 63, ln);
+            });
+            // the top block is finished: no later step reaches it
+            const int et = 64 * (g + NF - 1) + lane;
+            ring_st(rb, et < n ? et * SZ : -1, F[NF - 1]);
+#pragma unroll
+            for (int r = NF - 1; r >= 1; --r) F[r] = F[r - 1];
+            F[0] = bprev;
+        }
+#pragma unroll
+        for (int r = 1; r < NF; ++r) {
+            const int e = 64 * (r - 1) + lane;
+            ring_st(rb, e < n ? e * SZ : -1, F[r]);
+        }
+    }
+}
+
 }  // namespace
 
 // 'N' solve for bands with kl <= 128 and kl + ku <= 256; -1 otherwise.
@@ -236,9 +379,37 @@ int ring_gbtrs_launch(i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 ld, cons
     return -1;
 }
 
+// 'T' solve for bands with kl <= 128 and kl + ku <= 256; -1 otherwise.
+template <typename T>
+int ring_gbtrs_t_launch(i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 ld, const i64 *ipiv, T *B, i64 ldb,
+                        hipStream_t st) {
+    const i64 kd = kl + ku;
+    if (n >= (1ll << 30) || kl > 128 || kd > 256 || ld * n * (i64)sizeof(T) >= (1ll << 31) - (1ll << 24)) return -1;
+    const int nf = 1 + (int)cdiv(kl, 64), nb = 1 + (int)cdiv(kd, 64);
+    constexpr int PFD = sizeof(T) == 4 ? 32 : 16;
+    dim3 grid((unsigned)nrhs), block(64);
+#define BH_RING_T_CASE(NFv, NBv)                                                                               \
+    if (nf <= NFv && nb <= NBv) {                                                                              \
+        hipLaunchKernelGGL((gbtrs_t_ring<T, NFv, NBv, PFD>), grid, block, 0, st, (int)n, (int)kl, (int)ku, ab, \
+                           ld, ipiv, B, ldb);                                                                  \
+        BH_LAUNCH_CHECK();                                                                                     \
+        return BH_OK;                                                                                          \
+    }
+    BH_RING_T_CASE(2, 2)
+    BH_RING_T_CASE(2, 3)
+    BH_RING_T_CASE(3, 3)
+    BH_RING_T_CASE(3, 5)
+#undef BH_RING_T_CASE
+    return -1;
+}
+
 template int ring_gbtrs_launch<double>(i64, i64, i64, i64, const double *, i64, const i64 *, double *, i64,
                                        hipStream_t);
 template int ring_gbtrs_launch<float>(i64, i64, i64, i64, const float *, i64, const i64 *, float *, i64,
                                       hipStream_t);
+template int ring_gbtrs_t_launch<double>(i64, i64, i64, i64, const double *, i64, const i64 *, double *, i64,
+                                         hipStream_t);
+template int ring_gbtrs_t_launch<float>(i64, i64, i64, i64, const float *, i64, const i64 *, float *, i64,
+                                        hipStream_t);
 
 }  // namespace bh

@@ -291,15 +291,19 @@ gbtrs_t_wave(i64 n, i64 kl, i64 ku, const T *__restrict__ ab, i64 ld, const i64 
 template <typename T>
 int ring_gbtrs_launch(i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 ld, const i64 *ipiv, T *B, i64 ldb,
                       hipStream_t st);
+template <typename T>
+int ring_gbtrs_t_launch(i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 ld, const i64 *ipiv, T *B, i64 ldb,
+                        hipStream_t st);
 
 template <typename T>
 int wave_gbtrs_launch(bool trans_n, i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 ld, const i64 *ipiv, T *B,
                       i64 ldb, hipStream_t st) {
-    // 'N': the lane-ring kernel unless BANDEDHIP_TRS_KERNEL=shift asks for this file's
-    if (trans_n) {
+    // the lane-ring kernels unless BANDEDHIP_TRS_KERNEL=shift asks for this file's
+    {
         const char *e = getenv("BANDEDHIP_TRS_KERNEL");
         if (!(e && !strcmp(e, "shift"))) {
-            const int r = ring_gbtrs_launch<T>(n, kl, ku, nrhs, ab, ld, ipiv, B, ldb, st);
+            const int r = trans_n ? ring_gbtrs_launch<T>(n, kl, ku, nrhs, ab, ld, ipiv, B, ldb, st)
+                                  : ring_gbtrs_t_launch<T>(n, kl, ku, nrhs, ab, ld, ipiv, B, ldb, st);
             if (r != -1) return r;
         }
     }
