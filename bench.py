@@ -155,7 +155,7 @@ def bench_precision(bh, steps, warmup, shift=True):
 
 
 # ----------------------------------------------------------------- bandwidth-bound kernels
-def bench_bandwidth(bh, steps):
+def bench_bandwidth(bh, steps, skip_cpu=True):
     n = 1 << 20
     rng = np.random.default_rng(1)
     out = {}
@@ -205,6 +205,34 @@ def bench_bandwidth(bh, steps):
             nbytes = (17 + 9 + 25) * 8.0 * n
             out["gbmm_fp64_8x8_4x4_n2p20"] = {"GBps": nbytes / ms / 1e6, "frac_of_hbm_peak": nbytes / ms / 1e6 / HBM_PEAK_GBPS,
                                               "us": ms * 1e3, "bytes": nbytes}
+    if not skip_cpu:
+        # the reference's CPU path: one dgbmv per right-hand side, gbmm! per column
+        import scipy.linalg.blas as sb
+        from threadpoolctl import threadpool_limits
+
+        from oracle import oracle as O
+
+        Ah = np.asfortranarray(rng.standard_normal((17, n)))
+        xh = rng.standard_normal(n)
+        cpu = {}
+        for thr, tag in ((1, "1core"), (os.cpu_count() or 1, "allcores")):
+            with threadpool_limits(thr):
+                sb.dgbmv(n, n, 8, 8, 1.0, Ah, xh)
+                t0 = time.perf_counter()
+                for _ in range(10):
+                    sb.dgbmv(n, n, 8, 8, 1.0, Ah, xh)
+                us = (time.perf_counter() - t0) / 10 * 1e6
+                cpu[f"gbmv_fp64_us_{tag}"] = us
+                cpu[f"gbmv_fp64_GBps_{tag}"] = 152.0 * n / us / 1e3
+                cpu[f"gbmv_fp64_64rhs_us_{tag}"] = 64 * us  # 64 separate calls, A re-read each time
+        Bh = np.asfortranarray(rng.standard_normal((9, n)))
+        t0 = time.perf_counter()
+        O.gbmm(n, n, n, 1.0, Ah, 8, 8, Bh, 4, 4, 0.0, np.zeros((25, n), order="F"), 12, 12)
+        cpu["gbmm_fp64_us_1core"] = (time.perf_counter() - t0) * 1e6
+        cpu["cores"] = os.cpu_count()
+        cpu["sample"] = ("OpenBLAS 0.3.28 (scipy) dgbmv, mean of 10 calls per thread count; 64 RHS as 64 "
+                         "calls (what the reference's per-column mul! loop does); A*B by the C restatement of gbmm!")
+        out["cpu_bandwidth_n2p20"] = cpu
     return out
 
 
@@ -292,16 +320,45 @@ def bench_plain(bh, steps, skip_cpu):
     bh.sync()
     out = {"n": n, "l": l, "u": u, "us_gbmv": gpu_time(bh, f_mv, 50) * 1e3, "us_gbmm": gpu_time(bh, f_mm, 50) * 1e3,
            "us_solve": gpu_time(bh, f_sv, 5) * 1e3}
-    if not skip_cpu:
-        import scipy.linalg.lapack as la
+    # FP64 backward error of the GPU solve, and its pivots against OpenBLAS's
+    import scipy.linalg.blas as sb
+    import scipy.linalg.lapack as la
 
-        ab = np.zeros((2 * l + u + 1, n), order="F")
-        ab[l:, :] = A_h
+    x = f_sv().numpy()
+    r = b_h - sb.dgbmv(n, n, l, u, 1.0, A_h, x)
+    arow = np.zeros(n)  # row sums of |A|: ||A||_inf
+    for t in range(l + u + 1):
+        j = np.arange(n)
+        i = j - u + t
+        ok = (i >= 0) & (i < n)
+        np.add.at(arow, i[ok], np.abs(A_h[t, ok]))
+    out["backward_error_fp64"] = float(np.abs(r).max() / (arow.max() * np.abs(x).max() + np.abs(b_h).max()))
+    ab = np.zeros((2 * l + u + 1, n), order="F")
+    ab[l:, :] = A_h
+    F = bh.lu(A)
+    _, piv_cpu, _ = la.dgbtrf(ab, l, u)
+    out["pivots_equal_cpu"] = f"{int(np.sum(F.ipiv == piv_cpu + 1))}/{n}"
+    if not skip_cpu:
+        from threadpoolctl import threadpool_limits
+
+        from oracle import oracle as O
+
+        with threadpool_limits(1):
+            t0 = time.perf_counter()
+            for _ in range(20):
+                lub, piv, info = la.dgbtrf(ab, l, u)
+                la.dgbtrs(lub, l, u, b_h, piv)
+            out["cpu_us_solve_1core"] = (time.perf_counter() - t0) / 20 * 1e6
+            t0 = time.perf_counter()
+            for _ in range(200):
+                sb.dgbmv(n, n, l, u, 1.0, A_h, b_h)
+            out["cpu_us_gbmv_1core"] = (time.perf_counter() - t0) / 200 * 1e6
         t0 = time.perf_counter()
         for _ in range(20):
-            lub, piv, info = la.dgbtrf(ab, l, u)
-            la.dgbtrs(lub, l, u, b_h, piv)
-        out["cpu_us_solve_1core"] = (time.perf_counter() - t0) / 20 * 1e6
+            O.gbmm(n, n, n, 1.0, A_h, l, u, A_h, l, u, 0.0, np.zeros((15, n), order="F"), 8, 6)
+        out["cpu_us_gbmm_1core"] = (time.perf_counter() - t0) / 20 * 1e6
+        out["cpu_sample"] = ("OpenBLAS 0.3.28 (scipy) dgbmv / dgbtrf+dgbtrs on 1 thread; A*A by the C restatement "
+                             "of gbmm! (the reference's generic kernel), 1 thread")
     return out
 
 
@@ -364,13 +421,24 @@ def main():
     extra = {}
     if not args.headline_only:
         log("bandwidth-bound kernels")
-        extra.update(bench_bandwidth(bh, args.steps))
+        extra.update(bench_bandwidth(bh, args.steps, args.skip_cpu))
         log("batched systems")
         extra["batched_4096"] = bench_batched(bh, args.skip_cpu)
         log("differentiation")
         extra["adjoint_n262144_16x16"] = bench_adjoint(bh, args.steps, args.skip_cpu)
         log("plain case")
         extra["plain_n1e4_4x3"] = bench_plain(bh, args.steps, args.skip_cpu)
+        log("precision workload, no diagonal shift (pivoting active)")
+        P2 = bench_precision(bh, max(1, args.steps // 2), 1, shift=False)
+        ns = {"mixed_solves_per_s": 1000.0 / P2["ms_mixed"], "fp64_gpu_solves_per_s": 1000.0 / P2["ms_fp64"],
+              "refinement_iters": P2["iters"]}
+        if not args.skip_cpu:
+            xc2, pivc2, tc2 = cpu_reference_solve(P2["A_h"], P2["B_h"], P2["l"], P2["u"], os.cpu_count() or 1)
+            fe2 = lambda x: float(np.abs(x - xc2).max() / np.abs(xc2).max())
+            ns.update({"cpu_solves_per_s": 1.0 / tc2, "forward_error_mixed": fe2(P2["x_mixed"]),
+                       "forward_error_fp64_gpu": fe2(P2["x_fp64"]),
+                       "pivots_fp64_gpu_equal_cpu": f"{int(np.sum(P2['ipiv64'] == pivc2))}/{P2['n']}"})
+        extra["precision_noshift_n65536_128x128"] = ns
     else:
         extra.update(bench_bandwidth(bh, args.steps))
 
