@@ -3,7 +3,7 @@
 The arithmetic runs in libbandedhip.so (HIP kernels behind the C ABI in
 include/bandedhip.h); this package is the host-side mirror of the reference's
 operator interface for the path: BandedMatrix storage, mul!/A*x, A*B, lu,
-ldiv!/\\ and qr.
+ldiv!/\\, qr and the triangular band solve/product.
 """
 from ._lib import BandedHIPError, load as load_library, LIB_PATH
 from .device import DeviceArray, Timer, sync
@@ -14,6 +14,8 @@ from .mixed import MixedResult, MixedSolver, solve_mixed
 from .adjoint import band_outer_, loss_and_grad, mul_pullback, solve_pullback
 from .batch import BatchedGBSV, BatchGraph, c4_batch
 from .qr import BandedQR, QRPackedQ, qr, qr_, qr_ldiv_, qr_lmul_, qr_solve
+from .tribanded import (BandView, LowerTriangular, UnitLowerTriangular, UnitUpperTriangular, UpperTriangular, tbmv_,
+                        tbsv_)
 
 __all__ = [
     "BandedHIPError", "load_library", "LIB_PATH", "DeviceArray", "Timer", "sync", "BandedMatrix", "BandError",
@@ -21,4 +23,5 @@ __all__ = [
     "SingularException", "TransposeFact", "ldiv_", "lu", "lu_", "pad_for_lu", "solve", "MixedResult",
     "MixedSolver", "solve_mixed", "BandedQR", "QRPackedQ", "qr", "qr_", "qr_ldiv_", "qr_lmul_", "qr_solve",
     "BatchedGBSV", "BatchGraph", "c4_batch", "band_outer_", "loss_and_grad", "mul_pullback", "solve_pullback",
+    "BandView", "UpperTriangular", "LowerTriangular", "UnitUpperTriangular", "UnitLowerTriangular", "tbsv_", "tbmv_",
 ]

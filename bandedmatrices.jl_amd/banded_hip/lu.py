@@ -121,7 +121,12 @@ def lu(A: BandedMatrix, check: bool = True) -> BandedLU:
 
 
 def ldiv_(F, B: DeviceArray) -> DeviceArray:
-    """ldiv!(F, B) in place; F a BandedLU or transpose(F)."""
+    """ldiv!(F, B) in place; F a BandedLU or transpose(F), or a triangular view
+    of a BandedMatrix (UpperTriangular(A) etc., solved with tbsv)."""
+    from .tribanded import _Triangular
+
+    if isinstance(F, _Triangular):
+        return F.ldiv_(B)
     trans = "N"
     if isinstance(F, TransposeFact):
         F, trans = F.parent, "T"

@@ -162,6 +162,28 @@ int bh_dgbqrs(bh_int m, bh_int n, bh_int l, bh_int u, const double *D, bh_int ld
 int bh_sgbqrs(bh_int m, bh_int n, bh_int l, bh_int u, const float *D, bh_int ldd, const float *tau,
               bh_int nrhs, float *B, bh_int ldb);
 
+/* ---------------------------------------------------------- triangular */
+/* Triangular band solve x <- op(A)^{-1} x and product x <- op(A) x in BLAS
+ * xTBSV / xTBMV form: n x n triangle with k off-diagonals, uplo 'U' (A(i,j)
+ * at A[k+i-j + j*lda]) or 'L' (A(i,j) at A[i-j + j*lda]), diag 'U' for a unit
+ * diagonal that is never read, lda >= k+1, x with stride incx >= 1.
+ * A may point into a BandedMatrix's data (a row offset of its band) with the
+ * parent's leading dimension; only rows 0..k of each column are read.
+ * Solve 'N' equals OpenBLAS xtbsv bit for bit (division, then one fma per
+ * entry); 'T' sums the dots across a wavefront (agrees to rounding).
+ * The product sums each row's terms in order of distance from the diagonal;
+ * it takes n temporary elements from the stream-ordered allocator.
+ * Replace tbsv! / tbmv! (src/blas.jl:71-147) as triangular \ and * on banded
+ * data call them (src/tribanded.jl:47-54,75-96). */
+int bh_dtbsv(char uplo, char trans, char diag, bh_int n, bh_int k, const double *A, bh_int lda,
+             double *x, bh_int incx);
+int bh_stbsv(char uplo, char trans, char diag, bh_int n, bh_int k, const float *A, bh_int lda,
+             float *x, bh_int incx);
+int bh_dtbmv(char uplo, char trans, char diag, bh_int n, bh_int k, const double *A, bh_int lda,
+             double *x, bh_int incx);
+int bh_stbmv(char uplo, char trans, char diag, bh_int n, bh_int k, const float *A, bh_int lda,
+             float *x, bh_int incx);
+
 /* ------------------------------------------------------------- batched */
 /* Many independent square systems A_s x_s = b_s (kl_s <= 8, kl_s + ku_s <= 63)
  * factored and solved in one persistent launch: each of the W resident waves

@@ -13,6 +13,7 @@
  *   - LAPACK ?gbtf2 / ?gbtrf        (src/banded/BandedLU.jl:90-103)
  *   - LAPACK ?gbtrs 'N'/'T'         (src/banded/linalg.jl:24-47)
  *   - BLAS ?tbsv upper              (src/tribanded.jl:75-83, src/blas.jl:123-141)
+ *   - BLAS ?tbsv / ?tbmv, all cases (src/tribanded.jl:47-54,75-96, src/blas.jl:71-147)
  *   - banded Householder QR         (src/banded/bandedqr.jl:8-35) with the
  *     reflector!/reflectorApply! pair it imports from ArrayLayouts
  *   - Q / Q' application and solve  (src/banded/bandedqr.jl:38-94,175-206)
@@ -261,6 +262,90 @@ void bo_dtbsv_u(char trans, i64 n, i64 k, const double *a, i64 lda, double *x, i
 }
 void bo_stbsv_u(char trans, i64 n, i64 k, const float *a, i64 lda, float *x, i64 incx) {
     TBSV_BODY(float, fmaf)
+}
+
+/* General triangular band solve and product, x <- op(A)^{-1} x (xTBSV) and
+ * x <- op(A) x (xTBMV), as tbsv!/tbmv! call them (src/blas.jl:71-147) from
+ * triangular \ and * on banded data (src/tribanded.jl:47-54,75-96).
+ * uplo 'U': A(i,j) at a[k + i - j + j*lda] (diagonal in row k); 'L': A(i,j)
+ * at a[i - j + j*lda] (diagonal in row 0).  diag 'U' takes a unit diagonal
+ * that is never read.
+ * Solve 'N' restates the column-oriented BLAS sweep (upper: j descending,
+ * lower: j ascending): x_j = b_j / a_jj, then b_i = fma(a_ij, -x_j, b_i).
+ * Solve 'T' accumulates each dot over the off-diagonal entries in order of
+ * their distance from the diagonal, then divides.
+ * Product: y_i = d_i, d_i = a_ii x_i (x_i for a unit diagonal), then
+ * y_i = fma(a, x, y_i) over the off-diagonal terms of row i ('N') or column i
+ * ('T') in order of their distance t = 1..k from the diagonal. */
+#define TB_AT(t_, j_) (up ? a[IDX(lda, k - (t_), j_)] : a[IDX(lda, (t_), j_)])
+#define TBSV_GEN_BODY(T, FMA)                                                          \
+    const int up = (uplo == 'U' || uplo == 'u');                                         \
+    const int tr = !(trans == 'N' || trans == 'n');                                      \
+    const int unit = (diag == 'U' || diag == 'u');                                       \
+    const i64 dr = up ? k : 0;                                                           \
+    for (i64 q = 0; q < n; ++q) {                                                        \
+        /* 'N' upper and 'T' lower run j descending */                                   \
+        const i64 j = (up != tr) ? n - 1 - q : q;                                        \
+        /* the step touches entries after j (lower 'N', lower 'T') or before j */        \
+        const int fut = !up;                                                             \
+        const i64 tm = fut ? imin(k, n - 1 - j) : imin(k, j);                            \
+        if (!tr) {                                                                       \
+            T xj = x[j * incx];                                                          \
+            if (!unit) xj = xj / a[IDX(lda, dr, j)];                                     \
+            x[j * incx] = xj;                                                            \
+            const T nx = -xj;                                                            \
+            for (i64 t = 1; t <= tm; ++t) {                                              \
+                const i64 i = fut ? j + t : j - t;                                       \
+                x[i * incx] = FMA(TB_AT(t, j), nx, x[i * incx]);                         \
+            }                                                                            \
+        } else {                                                                         \
+            T part = 0;                                                                  \
+            for (i64 t = 1; t <= tm; ++t) {                                              \
+                const i64 i = fut ? j + t : j - t;                                       \
+                part = FMA(TB_AT(t, j), x[i * incx], part);                              \
+            }                                                                            \
+            const T v = x[j * incx] - part;                                              \
+            x[j * incx] = unit ? v : v / a[IDX(lda, dr, j)];                             \
+        }                                                                                \
+    }
+
+#define TBMV_GEN_BODY(T, FMA)                                                          \
+    const int up = (uplo == 'U' || uplo == 'u');                                         \
+    const int tr = !(trans == 'N' || trans == 'n');                                      \
+    const int unit = (diag == 'U' || diag == 'u');                                       \
+    const i64 dr = up ? k : 0;                                                           \
+    T *y = (T *)malloc(sizeof(T) * (size_t)(n > 0 ? n : 1));                             \
+    for (i64 i = 0; i < n; ++i) {                                                        \
+        T acc = unit ? x[i * incx] : a[IDX(lda, dr, i)] * x[i * incx];                   \
+        if (!tr && up) { /* row i of U: A(i, i+t) */                                     \
+            for (i64 t = 1; t <= imin(k, n - 1 - i); ++t)                                \
+                acc = FMA(a[IDX(lda, k - t, i + t)], x[(i + t) * incx], acc);            \
+        } else if (!tr) { /* row i of L: A(i, i-t) */                                    \
+            for (i64 t = 1; t <= imin(k, i); ++t)                                        \
+                acc = FMA(a[IDX(lda, t, i - t)], x[(i - t) * incx], acc);                \
+        } else if (up) { /* column i of U: A(i-t, i) */                                  \
+            for (i64 t = 1; t <= imin(k, i); ++t)                                        \
+                acc = FMA(a[IDX(lda, k - t, i)], x[(i - t) * incx], acc);                \
+        } else { /* column i of L: A(i+t, i) */                                          \
+            for (i64 t = 1; t <= imin(k, n - 1 - i); ++t)                                \
+                acc = FMA(a[IDX(lda, t, i)], x[(i + t) * incx], acc);                    \
+        }                                                                                \
+        y[i] = acc;                                                                      \
+    }                                                                                    \
+    for (i64 i = 0; i < n; ++i) x[i * incx] = y[i];                                      \
+    free(y);
+
+void bo_dtbsv(char uplo, char trans, char diag, i64 n, i64 k, const double *a, i64 lda, double *x, i64 incx) {
+    TBSV_GEN_BODY(double, fma)
+}
+void bo_stbsv(char uplo, char trans, char diag, i64 n, i64 k, const float *a, i64 lda, float *x, i64 incx) {
+    TBSV_GEN_BODY(float, fmaf)
+}
+void bo_dtbmv(char uplo, char trans, char diag, i64 n, i64 k, const double *a, i64 lda, double *x, i64 incx) {
+    TBMV_GEN_BODY(double, fma)
+}
+void bo_stbmv(char uplo, char trans, char diag, i64 n, i64 k, const float *a, i64 lda, float *x, i64 incx) {
+    TBMV_GEN_BODY(float, fmaf)
 }
 
 /* ------------------------------------------------------------- gbtrs */

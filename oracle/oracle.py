@@ -56,6 +56,10 @@ def lib():
             f = getattr(L, f"bo_{pre}tbsv_u")
             f.argtypes = [C.c_char, i64, i64, fp, i64, fp, i64]
             f.restype = None
+            for nm in ("tbsv", "tbmv"):
+                f = getattr(L, f"bo_{pre}{nm}")
+                f.argtypes = [C.c_char, C.c_char, C.c_char, i64, i64, fp, i64, fp, i64]
+                f.restype = None
         L.bo_dgbtf2_planted.argtypes = [C.c_int, i64, i64, i64, i64, _dp, i64, _ip]
         L.bo_dgbtf2_planted.restype = i64
         L.bo_dgbmm.argtypes = [i64, i64, i64, C.c_double, _dp, i64, i64, i64, _dp, i64, i64, i64,
@@ -148,6 +152,26 @@ def tbsv_u(trans, n, k, a, x):
     aa = _f(a)
     xx = np.ascontiguousarray(x, dtype=a.dtype).copy()
     getattr(lib(), f"bo_{pre}tbsv_u")(trans.encode(), n, k, _ptr(aa, P), aa.shape[0], _ptr(xx, P), 1)
+    return xx
+
+
+def tbsv(uplo, trans, diag, n, k, a, x):
+    """x <- op(A)^{-1} x for the triangular band A stored in a ((k+1) x n, BLAS layout)."""
+    pre, P = _pre(a.dtype)
+    aa = _f(a)
+    xx = np.ascontiguousarray(x, dtype=a.dtype).copy()
+    getattr(lib(), f"bo_{pre}tbsv")(uplo.encode(), trans.encode(), diag.encode(), n, k, _ptr(aa, P), aa.shape[0],
+                                    _ptr(xx, P), 1)
+    return xx
+
+
+def tbmv(uplo, trans, diag, n, k, a, x):
+    """x <- op(A) x for the triangular band A stored in a ((k+1) x n, BLAS layout)."""
+    pre, P = _pre(a.dtype)
+    aa = _f(a)
+    xx = np.ascontiguousarray(x, dtype=a.dtype).copy()
+    getattr(lib(), f"bo_{pre}tbmv")(uplo.encode(), trans.encode(), diag.encode(), n, k, _ptr(aa, P), aa.shape[0],
+                                    _ptr(xx, P), 1)
     return xx
 
 
