@@ -74,6 +74,10 @@ def _tb_call(name: str, uplo: str, trans: str, diag: str, m: int, k: int, A, x: 
     if n == 0:
         return x
     fn = f"bh_{_prefix(x.dtype)}{name}"
+    if name == "tbsv" and len(x.shape) == 2:  # all columns in one launch
+        _lib.call(fn + "_multi", uplo.encode(), trans.encode(), diag.encode(), n, k, V.ptr, V.ld, x.shape[1], x.ptr,
+                  x.ld)
+        return x
     for p in _columns(x):
         _lib.call(fn, uplo.encode(), trans.encode(), diag.encode(), n, k, V.ptr, V.ld, p, 1)
     return x
@@ -81,7 +85,8 @@ def _tb_call(name: str, uplo: str, trans: str, diag: str, m: int, k: int, A, x: 
 
 def tbsv_(uplo: str, trans: str, diag: str, m: int, k: int, A, x: DeviceArray) -> DeviceArray:
     """tbsv!(uplo, trans, diag, m, k, A, x): x <- op(T)^{-1} x in place (src/blas.jl:110-147).
-    A is the (>= k+1) x n triangular band data; a matrix x is solved column by column."""
+    A is the (>= k+1) x n triangular band data; the columns of a matrix x are
+    solved independently (one launch for all of them)."""
     return _tb_call("tbsv", uplo, trans, diag, m, k, A, x)
 
 

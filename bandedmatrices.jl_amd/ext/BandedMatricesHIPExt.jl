@@ -13,11 +13,12 @@
 #   ldiv!(F, B), F' \ B    LAPACK.gbtrs!                              src/banded/linalg.jl:24-47
 #   A \ b                  materialize!(::Ldiv{BandedColumns})        src/banded/linalg.jl:5-9,75
 #   qr / ldiv!(QR, B)      _banded_qr!, banded_qr_lmul!, tbsv!        src/banded/bandedqr.jl:1-214
+#   Tri(A) \ x, Tri(A) * x tbsv! / tbmv! on the triangle's band rows  src/tribanded.jl:19-26,47-54,75-96
 module BandedMatricesHIPExt
 
 using BandedMatrices, LinearAlgebra, ArrayLayouts, AMDGPU
 import BandedMatrices: BandedColumns, bandeddata, bandwidths, BandedLU, _BandedMatrix
-import ArrayLayouts: MemoryLayout, MulAdd, Ldiv, materialize!
+import ArrayLayouts: MemoryLayout, MulAdd, Ldiv, Lmul, TriangularLayout, triangulardata, materialize!
 import LinearAlgebra: BlasInt
 
 const libbh = joinpath(@__DIR__, "..", "libbandedhip.so")
@@ -61,7 +62,29 @@ for (T, p) in ((Float64, :d), (Float32, :s))
     gbmv = Symbol(:bh_, p, :gbmv); gbmv_multi = Symbol(:bh_, p, :gbmv_multi); gbmm = Symbol(:bh_, p, :gbmm)
     gbtrf = Symbol(:bh_, p, :gbtrf); gbtrs = Symbol(:bh_, p, :gbtrs); gbpad = Symbol(:bh_, p, :gbpad)
     gbqrf = Symbol(:bh_, p, :gbqrf); gbqrs = Symbol(:bh_, p, :gbqrs)
+    tbsv = Symbol(:bh_, p, :tbsv); tbmv = Symbol(:bh_, p, :tbmv)
     @eval begin
+        # tbsv!/tbmv!(uplo, trans, diag, m, k, bandeddata(Tri(A)), x) (src/blas.jl:71-147): the
+        # triangle's band is rows 1:u+1 ('U') or u+1:l+u+1 ('L') of A's band array
+        # (src/tribanded.jl:19-26), passed as a pointer into it with A's leading dimension.
+        function _tb!(op::Symbol, uplo::Char, trans::Char, diag::Char, A::HIPBanded{$T}, x::ROCVector{$T})
+            l, u = bandwidths(A); d = bandeddata(A); n = size(A, 1)
+            size(A, 2) == n || throw(DimensionMismatch("matrix is not square: dimensions are $(size(A))"))
+            length(x) == n || throw(DimensionMismatch("size of A is $n != length(x) = $(length(x))"))
+            k, off = uplo == 'U' ? (u, 0) : (l, u)
+            fn = op === :tbsv ? $(QuoteNode(tbsv)) : $(QuoteNode(tbmv))
+            with_stream() do
+                r = op === :tbsv ?
+                    ccall(($(QuoteNode(tbsv)), libbh), Cint,
+                          (UInt8, UInt8, UInt8, Int64, Int64, Ptr{$T}, Int64, Ptr{$T}, Int64),
+                          uplo, trans, diag, n, k, dptr(d) + off * sizeof($T), stride(d, 2), dptr(x), 1) :
+                    ccall(($(QuoteNode(tbmv)), libbh), Cint,
+                          (UInt8, UInt8, UInt8, Int64, Int64, Ptr{$T}, Int64, Ptr{$T}, Int64),
+                          uplo, trans, diag, n, k, dptr(d) + off * sizeof($T), stride(d, 2), dptr(x), 1)
+                chk(r, fn)
+            end
+            x
+        end
         # y <- α A x + β y   (BLAS.gbmv! in banded_gbmv!, src/generic/matmul.jl:21-23)
         function _gbmv!(trans::Char, α::$T, A::HIPBanded{$T}, x::ROCVector{$T}, β::$T, y::ROCVector{$T})
             l, u = bandwidths(A); d = bandeddata(A); m, n = size(A)
@@ -142,5 +165,11 @@ LinearAlgebra.ldiv!(F::BandedLU{T,<:HIPBanded{T}}, B::ROCVecOrMat{T}) where {T} 
 LinearAlgebra.ldiv!(F::Transpose{T,<:BandedLU{T,<:HIPBanded{T}}}, B::ROCVecOrMat{T}) where {T} = _gbtrs!('T', parent(F), B)
 LinearAlgebra.ldiv!(F::Adjoint{T,<:BandedLU{T,<:HIPBanded{T}}}, B::ROCVecOrMat{T}) where {T<:Real} = _gbtrs!('T', parent(F), B)
 materialize!(M::Ldiv{HIPBC}) = ldiv!(lu(M.A), M.B)
+# Triangular views: the layouts triangularlayout(Tri, ::BandedColumns) gives
+# (src/tribanded.jl:9-11), with UPLO in ('U','L') and UNIT in ('N','U').
+materialize!(M::Ldiv{<:TriangularLayout{UPLO,UNIT,HIPBC},HIPColumnMajor,<:Any,<:ROCVector}) where {UPLO,UNIT} =
+    _tb!(:tbsv, UPLO, 'N', UNIT, triangulardata(M.A), M.B)
+materialize!(M::Lmul{<:TriangularLayout{UPLO,UNIT,HIPBC},HIPColumnMajor,<:Any,<:ROCVector}) where {UPLO,UNIT} =
+    _tb!(:tbmv, UPLO, 'N', UNIT, triangulardata(M.A), M.B)
 
 end # module

@@ -362,6 +362,55 @@ def bench_plain(bh, steps, skip_cpu):
     return out
 
 
+def bench_triangular(bh, steps, skip_cpu):
+    """Triangular band solve and product (tbsv!/tbmv!), FP64, at the headline's
+    band (n = 65536, k = 128) and a narrow long one (n = 2^20, k = 8)."""
+    import scipy.linalg.blas as sb
+
+    out = {}
+    for n, k in ((65536, 128), (1 << 20, 8)):
+        rng = np.random.default_rng(1)
+        a_h = np.asfortranarray(rng.uniform(-1.0, 1.0, size=(k + 1, n)) * (0.5 / k))
+        a_h[k, :] = 1.0 + rng.random(n)  # upper: diagonal in row k
+        aL_h = np.asfortranarray(np.roll(a_h, -k, axis=0))  # lower: diagonal in row 0
+        x_h = rng.standard_normal(n)
+        A, AL = bh.DeviceArray.from_numpy(a_h), bh.DeviceArray.from_numpy(aL_h)
+        X = bh.DeviceArray.from_numpy(x_h)
+        r = {"n": n, "k": k}
+        for tag, uplo, trans, M in (("tbsv_UN", "U", "N", A), ("tbsv_LN", "L", "N", AL), ("tbsv_UT", "U", "T", A)):
+            f = lambda: bh.tbsv_(uplo, trans, "N", n, k, M, X)
+            f()
+            X.upload(x_h)
+            r[f"us_{tag}"] = gpu_time(bh, f, max(2, steps)) * 1e3
+            X.upload(x_h)
+        r["ns_per_step_tbsv_UN"] = r["us_tbsv_UN"] * 1e3 / n
+        XB = bh.DeviceArray.from_numpy(np.tile(x_h[:, None], (1, 64)))
+        fb = lambda: bh.tbsv_("U", "N", "N", n, k, A, XB)
+        fb()
+        r["us_tbsv_UN_64rhs"] = gpu_time(bh, fb, 2) * 1e3
+        del XB
+        fm = lambda: bh.tbmv_("U", "N", "N", n, k, A, X)
+        fm()
+        r["us_tbmv_UN"] = gpu_time(bh, fm, 20) * 1e3
+        r["tbmv_GBps_algorithmic"] = ((k + 1) * n + 2 * n) * 8 / (r["us_tbmv_UN"] * 1e-6) / 1e9
+        if not skip_cpu:
+            from threadpoolctl import threadpool_limits
+
+            with threadpool_limits(1):
+                t0 = time.perf_counter()
+                for _ in range(5):
+                    sb.dtbsv(k, a_h, x_h)
+                r["cpu_us_tbsv_UN_1core"] = (time.perf_counter() - t0) / 5 * 1e6
+                r["cpu_us_tbsv_UN_64rhs_1core"] = 64 * r["cpu_us_tbsv_UN_1core"]  # 64 independent calls
+                t0 = time.perf_counter()
+                for _ in range(5):
+                    sb.dtbmv(k, a_h, x_h)
+                r["cpu_us_tbmv_UN_1core"] = (time.perf_counter() - t0) / 5 * 1e6
+            r["cpu_sample"] = "OpenBLAS 0.3.28 (scipy) dtbsv / dtbmv, upper 'N', 1 thread, 5 calls each"
+        out[f"triangular_n{n}_k{k}"] = r
+    return out
+
+
 def traffic_probe(bh, name):
     """A few launches of one kernel plus a 256 MiB device copy, for counter runs."""
     import ctypes as C
@@ -400,6 +449,8 @@ def main():
     ap.add_argument("--headline-only", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--traffic-probe", default=None)
+    ap.add_argument("--only", choices=["triangular"], default=None,
+                    help="run one side workload and print its figures as one JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -413,6 +464,9 @@ def main():
         raise RuntimeError(L.bh_last_error().decode())
     if args.traffic_probe:
         traffic_probe(bh, args.traffic_probe)
+        return
+    if args.only == "triangular":
+        print(json.dumps(bench_triangular(bh, args.steps, args.skip_cpu)), flush=True)
         return
 
     log("precision workload")
@@ -428,6 +482,8 @@ def main():
         extra["adjoint_n262144_16x16"] = bench_adjoint(bh, args.steps, args.skip_cpu)
         log("plain case")
         extra["plain_n1e4_4x3"] = bench_plain(bh, args.steps, args.skip_cpu)
+        log("triangular band solve and product")
+        extra.update(bench_triangular(bh, args.steps, args.skip_cpu))
         log("precision workload, no diagonal shift (pivoting active)")
         P2 = bench_precision(bh, max(1, args.steps // 2), 1, shift=False)
         ns = {"mixed_solves_per_s": 1000.0 / P2["ms_mixed"], "fp64_gpu_solves_per_s": 1000.0 / P2["ms_fp64"],

@@ -179,6 +179,14 @@ int bh_dtbsv(char uplo, char trans, char diag, bh_int n, bh_int k, const double 
              double *x, bh_int incx);
 int bh_stbsv(char uplo, char trans, char diag, bh_int n, bh_int k, const float *A, bh_int lda,
              float *x, bh_int incx);
+/* The same solve for the nrhs columns of X (n x nrhs, leading dimension ldx),
+ * one wavefront per column in one launch; each column's result equals the
+ * single-vector call's bit for bit.  Replaces the column loop of tbsv! calls
+ * that ldiv!(Tri(A), B) makes for a matrix B (src/tribanded.jl:75-96). */
+int bh_dtbsv_multi(char uplo, char trans, char diag, bh_int n, bh_int k, const double *A, bh_int lda,
+                   bh_int nrhs, double *X, bh_int ldx);
+int bh_stbsv_multi(char uplo, char trans, char diag, bh_int n, bh_int k, const float *A, bh_int lda,
+                   bh_int nrhs, float *X, bh_int ldx);
 int bh_dtbmv(char uplo, char trans, char diag, bh_int n, bh_int k, const double *A, bh_int lda,
              double *x, bh_int incx);
 int bh_stbmv(char uplo, char trans, char diag, bh_int n, bh_int k, const float *A, bh_int lda,

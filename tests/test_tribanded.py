@@ -173,6 +173,10 @@ def test_argument_errors_blas_style():
         assert fn(b"U", b"N", b"N", 4, 3, None, 3, None, 1) == -7
         assert fn(b"U", b"N", b"N", 4, 1, None, 2, None, 0) == -9
         assert fn(b"L", b"T", b"U", 0, 1, None, 2, None, 1) == 0  # empty: nothing to do
+    for fn in (L.bh_dtbsv_multi, L.bh_stbsv_multi):
+        assert fn(b"U", b"N", b"N", 4, 1, None, 2, -1, None, 4) == -8
+        assert fn(b"U", b"N", b"N", 4, 1, None, 2, 2, None, 3) == -10
+        assert fn(b"U", b"N", b"N", 4, 1, None, 2, 0, None, 4) == 0
 
 
 # ------------------------------------------------------------------ GPU (C ABI)
@@ -273,6 +277,28 @@ def test_gpu_triangular_views_of_banded_matrix(bh, oracle, cls):
     X = rng.standard_normal((n, 3))
     got = T.ldiv_(bh.DeviceArray.from_numpy(X), trans="T").numpy()
     assert np.allclose(got, np.linalg.solve(dense.T, X), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("uplo,trans,diag", list(itertools.product(UPLO, TRANS, DIAG)))
+def test_gpu_tbsv_multi_equals_single(bh, oracle, uplo, trans, diag):
+    """All columns in one launch: each equals the single-vector call bit for bit
+    (and the oracle, bitwise for 'N'), with ldx > n."""
+    rng = np.random.default_rng(16)
+    for n, k, nrhs in [(257, 9, 5), (1000, 130, 3), (90, 400, 2)]:
+        a = tri_band(rng, n, k, uplo, diag)
+        X = rng.standard_normal((n + 3, nrhs))
+        A = bh.DeviceArray.from_numpy(a)
+        DX = bh.DeviceArray.from_numpy(X)
+        bh._lib.call("bh_dtbsv_multi", uplo.encode(), trans.encode(), diag.encode(), n, k, A.ptr, A.ld, nrhs,
+                     DX.ptr, n + 3)
+        got = DX.numpy()
+        assert bits_equal(got[n:], X[n:]), "rows below n were written"
+        for c in range(nrhs):
+            single = _gpu_tb(bh, "tbsv", uplo, trans, diag, n, k, a, X[:n, c])
+            assert bits_equal(got[:n, c], single), (n, k, c)
+            if trans == "N":
+                assert bits_equal(single, oracle.tbsv(uplo, trans, diag, n, k, a, X[:n, c]))
 
 
 @pytest.mark.gpu
