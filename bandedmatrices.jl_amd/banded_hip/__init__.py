@@ -7,8 +7,8 @@ ldiv!/\\, qr and the triangular band solve/product.
 """
 from ._lib import BandedHIPError, load as load_library, LIB_PATH
 from .device import DeviceArray, Timer, sync
-from .bandedmatrix import (BandedMatrix, BandError, DimensionMismatch, _BandedMatrix, brand, gbmm, mul,
-                           prodbandwidths)
+from .bandedmatrix import (BandedMatrix, BandError, DimensionMismatch, _BandedMatrix, axpy_, brand, convert, copyto_,
+                           gbmm, mul, prodbandwidths)
 from .lu import BandedLU, SingularException, TransposeFact, ldiv_, lu, lu_, pad_for_lu, solve
 from .mixed import MixedResult, MixedSolver, solve_mixed
 from .adjoint import band_outer_, loss_and_grad, mul_pullback, solve_pullback
@@ -23,5 +23,5 @@ __all__ = [
     "SingularException", "TransposeFact", "ldiv_", "lu", "lu_", "pad_for_lu", "solve", "MixedResult",
     "MixedSolver", "solve_mixed", "BandedQR", "QRPackedQ", "qr", "qr_", "qr_ldiv_", "qr_lmul_", "qr_solve",
     "BatchedGBSV", "BatchGraph", "c4_batch", "band_outer_", "loss_and_grad", "mul_pullback", "solve_pullback",
-    "BandView", "UpperTriangular", "LowerTriangular", "UnitUpperTriangular", "UnitLowerTriangular", "tbsv_", "tbmv_",
+    "axpy_", "convert", "copyto_", "BandView", "UpperTriangular", "LowerTriangular", "UnitUpperTriangular", "UnitLowerTriangular", "tbsv_", "tbmv_",
 ]

@@ -8,6 +8,9 @@ Names, argument meaning and errors follow BandedMatrices.jl:
   A*B (gbmm!)                    src/generic/matmul.jl:1-6,177-189; src/banded/gbmm.jl:207-293
   prodbandwidths                 src/generic/utils.jl:19-21
   BandError                      src/generic/Band.jl:75-88
+  view(A, kr, jr) (banded view)  src/banded/BandedMatrix.jl:918-954 (bandshift, bandwidths, bandeddata)
+  lmul!/rmul!, axpy!, copyto!    src/generic/broadcast.jl:942-1020, src/banded/BandedMatrix.jl:124-165
+  negative-band products         src/generic/matmul.jl:41-59, src/banded/gbmm.jl:231-279
 Data lives in HBM (``DeviceArray``); every operation calls libbandedhip.
 """
 from __future__ import annotations
@@ -97,6 +100,43 @@ class BandedMatrix:
     def copy(self) -> "BandedMatrix":
         return BandedMatrix(self.data.copy(), self.m, self.l, self.u)
 
+    def view(self, kr: range, jr: range) -> "BandedMatrix":
+        """view(A, kr, jr) for unit ranges (0-based, step 1): a banded view sharing
+        A's band array (BandedSubBandedMatrix, src/banded/BandedMatrix.jl:918-954).
+        bandshift = first(kr) - first(jr); bandwidths = min.(size .- 1, (l - s, u + s));
+        the view's band rows are U-u .. U+l of A's (0-based), U = u + s, columns jr."""
+        m, n = self.shape
+        if len(kr) and (kr.start < 0 or kr.stop > m) or len(jr) and (jr.start < 0 or jr.stop > n):
+            raise IndexError(f"view {kr}, {jr} of a {m}x{n} matrix")
+        lv, uv, row0, h = view_geometry(self.l, self.u, kr, jr)
+        return BandedMatrix(self.data.view(row0, jr.start if len(jr) else 0, (h, len(jr))), len(kr), lv, uv)
+
+    # element-wise (the reference's fast paths on bandeddata)
+    def lmul_(self, alpha) -> "BandedMatrix":
+        """lmul!(alpha, A): every stored entry times alpha (src/generic/broadcast.jl:942-949)."""
+        m, n = self.shape
+        _lib.call(f"bh_{_prefix(self.dtype)}gbscal", m, n, self.l, self.u, alpha, self.data.ptr, self.data.ld,
+                  0, m, 0, n, 0)
+        return self
+
+    rmul_ = lmul_
+
+    def __mul__(self, alpha):
+        if not np.isscalar(alpha):
+            return NotImplemented
+        return self.copy().lmul_(alpha)
+
+    __rmul__ = __mul__
+
+    def __add__(self, other):
+        return _plus(self, other, 1.0)
+
+    def __sub__(self, other):
+        return _plus(self, other, -1.0)
+
+    def __neg__(self):
+        return self * -1.0
+
     def __matmul__(self, other):
         if isinstance(other, BandedMatrix):
             return gbmm(self, other)
@@ -112,6 +152,20 @@ class BandedMatrix:
         return f"BandedMatrix({self.m}x{self.shape[1]}, bands={self.bandwidths()}, {self.dtype})"
 
 
+def view_geometry(l: int, u: int, kr: range, jr: range):
+    """Bandwidths (lv, uv) of view(A, kr, jr) and the first row / row count of its
+    band inside A's band array (src/banded/BandedMatrix.jl:918-954, 0-based):
+    s = first(kr) - first(jr), (lv, uv) = min.(size .- 1, (l - s, u + s)), rows
+    U-uv .. U+lv with U = u + s."""
+    if kr.step != 1 or jr.step != 1:
+        raise ValueError("banded views take unit ranges")
+    sft = kr.start - jr.start
+    lv = min(len(kr) - 1, l - sft)
+    uv = min(len(jr) - 1, u + sft)
+    h = max(0, lv + uv + 1)
+    return lv, uv, (u + sft - uv) if h else 0, h
+
+
 def _BandedMatrix(data: DeviceArray, m: int, l: int, u: int) -> BandedMatrix:
     return BandedMatrix(data, m, l, u)
 
@@ -123,6 +177,56 @@ def brand(dtype, n, m, l, u, *, seed=0, normal=False) -> BandedMatrix:
     h = max(0, l + u + 1)
     d = rng.standard_normal((h, m)) if normal else rng.random((h, m))
     return BandedMatrix.from_host(np.asfortranarray(d.astype(dtype)), n, l, u)
+
+
+def axpy_(a, X: BandedMatrix, Y: BandedMatrix) -> BandedMatrix:
+    """axpy!(a, X, Y): Y = a*X + Y on the bands X and Y share; BandError if X has
+    a nonzero outside Y's bands (banded_generic_axpy!, src/generic/broadcast.jl:978-1020)."""
+    if X.shape != Y.shape:
+        raise DimensionMismatch(f"X has size {X.shape} but Y has size {Y.shape}")
+    if X.dtype != Y.dtype:
+        raise TypeError("mixed element types")
+    m, n = X.shape
+    try:
+        _lib.call(f"bh_{_prefix(X.dtype)}gbaxpy", m, n, a, X.data.ptr, X.data.ld, X.l, X.u, Y.data.ptr, Y.data.ld,
+                  Y.l, Y.u)
+    except _lib.BandedHIPError as e:
+        if e.status == -1003:
+            raise BandError(str(e)) from None
+        raise
+    return Y
+
+
+def copyto_(B: BandedMatrix, A: BandedMatrix) -> BandedMatrix:
+    """copyto!(B, A) between banded matrices of the same size: A's entries copied,
+    the rest of B's band zeroed; BandError if A has a nonzero outside B's bands."""
+    if A.shape != B.shape:
+        raise DimensionMismatch(f"A has size {A.shape} but B has size {B.shape}")
+    if A.dtype != B.dtype:
+        raise TypeError("mixed element types")
+    m, n = A.shape
+    try:
+        _lib.call(f"bh_{_prefix(A.dtype)}gbcopy", m, n, A.data.ptr, A.data.ld, A.l, A.u, B.data.ptr, B.data.ld,
+                  B.l, B.u)
+    except _lib.BandedHIPError as e:
+        if e.status == -1003:
+            raise BandError(str(e)) from None
+        raise
+    return B
+
+
+def convert(A: BandedMatrix, bands) -> BandedMatrix:
+    """BandedMatrix(A, (l, u)): a new matrix with the given bandwidths holding A."""
+    B = BandedMatrix.undef(A.shape, bands, A.dtype)
+    return copyto_(B, A)
+
+
+def _plus(A: BandedMatrix, B: BandedMatrix, sign: float) -> BandedMatrix:
+    """A + B / A - B with bandwidths max.(bandwidths(A), bandwidths(B))."""
+    if not isinstance(B, BandedMatrix):
+        return NotImplemented
+    C = convert(A, (max(A.l, B.l), max(A.u, B.u)))
+    return axpy_(sign, B, C)
 
 
 def prodbandwidths(*As):
@@ -215,7 +319,14 @@ def gbmm(A: BandedMatrix, B: BandedMatrix, C: BandedMatrix | None = None, alpha=
     if C.shape != (n, m):
         raise DimensionMismatch(f"C is {C.shape}, expected {(n, m)}")
     if min(A.l, A.u, B.l, B.u, C.l, C.u) < 0:
-        raise NotImplementedError("negative bandwidths in banded*banded")
+        # Negative bandwidths: the product runs on views that drop the zero
+        # rows/columns (src/banded/gbmm.jl:231-251).  A view's band slots past
+        # its own rows are entries of the parent, so beta is applied to C once
+        # here (0 when beta == 0) and the pruned product adds alpha*A*B with
+        # beta = 1, which leaves every slot it scales unchanged.
+        _scale_block(C, 0, n, 0, m, beta)
+        _gbmm_views(A, B, C, alpha)
+        return C
     p = _prefix(A.dtype)
     try:
         _lib.call(f"bh_{p}gbmm", n, k, m, alpha, A.data.ptr, A.data.ld, A.l, A.u, B.data.ptr, B.data.ld, B.l, B.u,
@@ -225,3 +336,49 @@ def gbmm(A: BandedMatrix, B: BandedMatrix, C: BandedMatrix | None = None, alpha=
             raise BandError(str(e)) from None
         raise
     return C
+
+
+def _rows_from(A: BandedMatrix, s: int) -> BandedMatrix:
+    """view(A, s:m, :): the same band array, bands (l - s, u + s)."""
+    return BandedMatrix(A.data, A.m - s, A.l - s, A.u + s)
+
+
+def _cols_from(A: BandedMatrix, s: int) -> BandedMatrix:
+    """view(A, :, s:n): the band array from column s, bands (l + s, u - s)."""
+    h, n = A.data.shape
+    return BandedMatrix(A.data.view(0, s, (h, n - s)), A.m, A.l + s, A.u - s)
+
+
+def _scale_block(C: BandedMatrix, r0, r1, c0, c1, beta) -> None:
+    """C[r0:r1, c0:c1] = beta*C there (0 when beta == 0): _fill_lmul! on a view."""
+    m, n = C.shape
+    if r1 > r0 and c1 > c0 and C.l + C.u + 1 > 0:
+        _lib.call(f"bh_{_prefix(C.dtype)}gbscal", m, n, C.l, C.u, beta, C.data.ptr, C.data.ld, r0, r1, c0, c1, 1)
+
+
+def _gbmm_views(A: BandedMatrix, B: BandedMatrix, C: BandedMatrix, alpha) -> None:
+    """C += alpha*A*B when a bandwidth of A, B or C is negative: zero rows and
+    columns are dropped through views until every bandwidth is >= 0, as the
+    band-pruning branches of gbmm! do (src/banded/gbmm.jl:231-251)."""
+    (Am, An), Bn = A.shape, B.shape[1]
+    if -A.l > A.u or -B.l > B.u or C.l + C.u < 0 or Am == 0 or An == 0 or Bn == 0:
+        return  # a factor without bands (or nothing to add)
+    if A.l < 0:  # A's first -l columns are zero: drop them and B's first -l rows
+        s = min(-A.l, An)
+        return _gbmm_views(_cols_from(A, s), _rows_from(B, s), C, alpha)
+    if A.u < 0:  # A's first -u rows are zero, and so are C's rows there
+        s = min(-A.u, Am)
+        return _gbmm_views(_rows_from(A, s), B, _rows_from(C, s), alpha)
+    if B.l < 0:  # B's first -l columns are zero, and so are C's columns there
+        s = min(-B.l, Bn)
+        return _gbmm_views(A, _cols_from(B, s), _cols_from(C, s), alpha)
+    if B.u < 0:  # B's first -u rows are zero: drop them and A's first -u columns
+        s = min(-B.u, An)
+        return _gbmm_views(_cols_from(A, s), _rows_from(B, s), C, alpha)
+    if C.u < 0:  # C stores nothing in its first -u rows
+        s = min(-C.u, Am)
+        return _gbmm_views(_rows_from(A, s), B, _rows_from(C, s), alpha)
+    if C.l < 0:  # C stores nothing in its first -l columns
+        s = min(-C.l, Bn)
+        return _gbmm_views(A, _cols_from(B, s), _cols_from(C, s), alpha)
+    gbmm(A, B, C, alpha, 1.0)

@@ -23,6 +23,8 @@ class DeviceArray:
         _lib.call("bh_malloc", C.byref(p), self.nbytes)
         self._ptr = p.value
         self._owner = True
+        self._ld = None
+        self._parent = None
 
     @property
     def ptr(self) -> int:
@@ -31,7 +33,32 @@ class DeviceArray:
     @property
     def ld(self) -> int:
         """Leading dimension (rows) for 2-d arrays."""
+        if self._ld is not None:
+            return self._ld
         return max(1, self.shape[0]) if self.shape else 1
+
+    @property
+    def contiguous(self) -> bool:
+        return len(self.shape) < 2 or self.ld == max(1, self.shape[0])
+
+    def view(self, row0: int, col0: int, shape) -> "DeviceArray":
+        """A non-owning window (rows row0.., columns col0..) of a 2-d array with
+        this array's leading dimension, like view(A, rows, cols) of a strided matrix."""
+        if len(self.shape) != 2:
+            raise ValueError("view needs a 2-d array")
+        nr, nc = (int(s) for s in shape)
+        if row0 < 0 or col0 < 0 or nr < 0 or nc < 0 or (nr and row0 + nr > self.shape[0]) or \
+                (nc and col0 + nc > self.shape[1]):
+            raise IndexError(f"view {row0}:{row0 + nr}, {col0}:{col0 + nc} of {self.shape}")
+        v = DeviceArray.__new__(DeviceArray)
+        v.shape = (nr, nc)
+        v.dtype = self.dtype
+        v.nbytes = nr * nc * self.dtype.itemsize
+        v._ptr = self.ptr + (row0 + col0 * self.ld) * self.dtype.itemsize
+        v._owner = False
+        v._ld = self.ld
+        v._parent = self  # keeps the allocation alive
+        return v
 
     @classmethod
     def from_numpy(cls, a: np.ndarray, dtype=None) -> "DeviceArray":
@@ -49,12 +76,22 @@ class DeviceArray:
         return d
 
     def numpy(self) -> np.ndarray:
+        if not self.contiguous:
+            nr, nc = self.shape
+            if self.nbytes == 0:
+                return np.empty(self.shape, dtype=self.dtype, order="F")
+            span = (nc - 1) * self.ld + nr
+            buf = np.empty(nc * self.ld, dtype=self.dtype)
+            _lib.call("bh_memcpy_dtoh", buf.ctypes.data, self.ptr, span * self.dtype.itemsize)
+            return np.asfortranarray(buf.reshape((self.ld, nc), order="F")[:nr, :])
         h = np.empty(self.shape, dtype=self.dtype, order="F")
         if self.nbytes:
             _lib.call("bh_memcpy_dtoh", h.ctypes.data, self.ptr, self.nbytes)
         return h
 
     def copy(self) -> "DeviceArray":
+        if not self.contiguous:
+            return DeviceArray.from_numpy(self.numpy())
         d = DeviceArray(self.shape, self.dtype)
         if self.nbytes:
             _lib.call("bh_memcpy_dtod", d.ptr, self.ptr, self.nbytes)
@@ -64,6 +101,11 @@ class DeviceArray:
         h = np.asfortranarray(a, dtype=self.dtype)
         if h.nbytes != self.nbytes:
             raise ValueError(f"size mismatch: {h.shape} vs {self.shape}")
+        if not self.contiguous:
+            for c in range(self.shape[1]):
+                col = np.ascontiguousarray(h[:, c])
+                _lib.call("bh_memcpy_htod", self.ptr + c * self.ld * self.dtype.itemsize, col.ctypes.data, col.nbytes)
+            return
         if self.nbytes:
             _lib.call("bh_memcpy_htod", self.ptr, h.ctypes.data, self.nbytes)
 

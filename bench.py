@@ -411,6 +411,64 @@ def bench_triangular(bh, steps, skip_cpu):
     return out
 
 
+def bench_elementwise(bh, steps, skip_cpu):
+    """Element-wise operations on band data (lmul!, axpy!, copyto! with a band
+    change), FP64, n = 2^20, bands (8, 8): HBM-bound, so reported as algorithmic
+    GB/s (every in-matrix slot read and written once)."""
+    n, l, u = 1 << 20, 8, 8
+    h = l + u + 1
+    rng = np.random.default_rng(2)
+    D = np.asfortranarray(rng.standard_normal((h, n)))
+    # three independent sets, used in turn, so no launch finds its operands in
+    # the 256 MB last-level cache (each set holds 460 MB)
+    sets = [(bh.BandedMatrix.from_host(D, n, l, u), bh.BandedMatrix.from_host(D, n, l, u),
+             bh.BandedMatrix.undef((n, n), (l + 2, u + 2))) for _ in range(3)]
+    it = {"k": 0}
+
+    def nxt():
+        it["k"] += 1
+        return sets[it["k"] % 3]
+
+    def f_ax():
+        X, Y, _ = nxt()
+        bh.axpy_(1e-3, X, Y)
+
+    def f_sc():
+        _, Y, _ = nxt()
+        Y.lmul_(0.999)
+
+    def f_cp():
+        X, _, W = nxt()
+        bh.copyto_(W, X)
+
+    for f in (f_ax, f_sc, f_cp):
+        f()
+    reps = max(21, steps * 3 * 2)
+    slots = h * n * 8
+    r = {"n": n, "l": l, "u": u}
+    r["us_axpy"] = gpu_time(bh, f_ax, reps) * 1e3
+    r["axpy_GBps"] = 3 * slots / (r["us_axpy"] * 1e-6) / 1e9
+    r["us_lmul"] = gpu_time(bh, f_sc, reps) * 1e3
+    r["lmul_GBps"] = 2 * slots / (r["us_lmul"] * 1e-6) / 1e9
+    r["us_copy_band_change"] = gpu_time(bh, f_cp, reps) * 1e3
+    r["copy_GBps"] = (slots + (h + 4) * n * 8) / (r["us_copy_band_change"] * 1e-6) / 1e9
+    if not skip_cpu:
+        from threadpoolctl import threadpool_limits
+
+        Yh = D.copy()
+        with threadpool_limits(1):
+            t0 = time.perf_counter()
+            for _ in range(10):
+                Yh += 1e-3 * D  # a*X + Y on the band array, as banded_generic_axpy! loops over it
+            r["cpu_us_axpy_1core"] = (time.perf_counter() - t0) / 10 * 1e6
+            t0 = time.perf_counter()
+            for _ in range(10):
+                Yh *= 0.999
+            r["cpu_us_lmul_1core"] = (time.perf_counter() - t0) / 10 * 1e6
+        r["cpu_sample"] = "numpy on the 17 x 2^20 band array, 1 thread, 10 calls each"
+    return r
+
+
 def traffic_probe(bh, name):
     """A few launches of one kernel plus a 256 MiB device copy, for counter runs."""
     import ctypes as C
@@ -449,7 +507,7 @@ def main():
     ap.add_argument("--headline-only", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--traffic-probe", default=None)
-    ap.add_argument("--only", choices=["triangular"], default=None,
+    ap.add_argument("--only", choices=["triangular", "elementwise"], default=None,
                     help="run one side workload and print its figures as one JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -468,6 +526,9 @@ def main():
     if args.only == "triangular":
         print(json.dumps(bench_triangular(bh, args.steps, args.skip_cpu)), flush=True)
         return
+    if args.only == "elementwise":
+        print(json.dumps(bench_elementwise(bh, args.steps, args.skip_cpu)), flush=True)
+        return
 
     log("precision workload")
     P = bench_precision(bh, args.steps, args.warmup, shift=not args.noshift)
@@ -484,6 +545,8 @@ def main():
         extra["plain_n1e4_4x3"] = bench_plain(bh, args.steps, args.skip_cpu)
         log("triangular band solve and product")
         extra.update(bench_triangular(bh, args.steps, args.skip_cpu))
+        log("element-wise band operations")
+        extra["elementwise_n2p20_8x8"] = bench_elementwise(bh, args.steps, args.skip_cpu)
         log("precision workload, no diagonal shift (pivoting active)")
         P2 = bench_precision(bh, max(1, args.steps // 2), 1, shift=False)
         ns = {"mixed_solves_per_s": 1000.0 / P2["ms_mixed"], "fp64_gpu_solves_per_s": 1000.0 / P2["ms_fp64"],

@@ -162,6 +162,36 @@ int bh_dgbqrs(bh_int m, bh_int n, bh_int l, bh_int u, const double *D, bh_int ld
 int bh_sgbqrs(bh_int m, bh_int n, bh_int l, bh_int u, const float *D, bh_int ldd, const float *tau,
               bh_int nrhs, float *B, bh_int ldb);
 
+/* -------------------------------------------------------- element-wise */
+/* Scale the stored entries A(i,j), r0 <= i < r1, c0 <= j < c1 (0-based,
+ * half-open), of an m x n band (kl, ku, either may be negative with
+ * kl + ku >= -1): A(i,j) <- alpha*A(i,j); with strong_zero != 0 and alpha == 0
+ * they are set to 0 (NaN and Inf included).  Slots outside the matrix are not
+ * touched.  Replaces lmul!(α, A) / rmul!(A, α) on a BandedMatrix
+ * (src/generic/broadcast.jl:942-958; strong_zero = 0) and the partial
+ * _fill_lmul!(β, view(C, ...)) of the band-pruned gbmm! (src/banded/gbmm.jl:231-279). */
+int bh_dgbscal(bh_int m, bh_int n, bh_int kl, bh_int ku, double alpha, double *A, bh_int lda,
+               bh_int r0, bh_int r1, bh_int c0, bh_int c1, int strong_zero);
+int bh_sgbscal(bh_int m, bh_int n, bh_int kl, bh_int ku, float alpha, float *A, bh_int lda,
+               bh_int r0, bh_int r1, bh_int c0, bh_int c1, int strong_zero);
+/* Y <- alpha*X + Y for m x n bands X (xl, xu) and Y (yl, yu) over the bands
+ * they share, each entry alpha*x rounded, then added.  A nonzero of X outside
+ * Y's bands returns BH_ERR_BANDERROR before Y is modified.
+ * Replaces banded_generic_axpy! (src/generic/broadcast.jl:978-1020). */
+int bh_dgbaxpy(bh_int m, bh_int n, double alpha, const double *X, bh_int ldx, bh_int xl, bh_int xu,
+               double *Y, bh_int ldy, bh_int yl, bh_int yu);
+int bh_sgbaxpy(bh_int m, bh_int n, float alpha, const float *X, bh_int ldx, bh_int xl, bh_int xu,
+               float *Y, bh_int ldy, bh_int yl, bh_int yu);
+/* B <- A between m x n bands of different bandwidths: B's entries on A's bands
+ * copied bit for bit, the rest of B's band set to 0; a nonzero of A outside
+ * B's bands returns BH_ERR_BANDERROR before B is modified.  Replaces
+ * copyto!(B, A) / BandedMatrix(A, (l, u)) on banded data
+ * (src/banded/BandedMatrix.jl:124-165). */
+int bh_dgbcopy(bh_int m, bh_int n, const double *A, bh_int lda, bh_int al, bh_int au, double *B,
+               bh_int ldb, bh_int bl, bh_int bu);
+int bh_sgbcopy(bh_int m, bh_int n, const float *A, bh_int lda, bh_int al, bh_int au, float *B,
+               bh_int ldb, bh_int bl, bh_int bu);
+
 /* ---------------------------------------------------------- triangular */
 /* Triangular band solve x <- op(A)^{-1} x and product x <- op(A) x in BLAS
  * xTBSV / xTBMV form: n x n triangle with k off-diagonals, uplo 'U' (A(i,j)
