@@ -14,6 +14,8 @@
 #   A \ b                  materialize!(::Ldiv{BandedColumns})        src/banded/linalg.jl:5-9,75
 #   qr / ldiv!(QR, B)      _banded_qr!, banded_qr_lmul!, tbsv!        src/banded/bandedqr.jl:1-214
 #   Tri(A) \ x, Tri(A) * x tbsv! / tbmv! on the triangle's band rows  src/tribanded.jl:19-26,47-54,75-96
+#   lmul!/rmul!, axpy!     _banded_lmul!/_rmul!, _banded_axpy!         src/generic/broadcast.jl:942-1020
+#   copyto!(B, A) banded   copyto_bandeddata!                          src/banded/BandedMatrix.jl:124-130
 module BandedMatricesHIPExt
 
 using BandedMatrices, LinearAlgebra, ArrayLayouts, AMDGPU
@@ -63,7 +65,42 @@ for (T, p) in ((Float64, :d), (Float32, :s))
     gbtrf = Symbol(:bh_, p, :gbtrf); gbtrs = Symbol(:bh_, p, :gbtrs); gbpad = Symbol(:bh_, p, :gbpad)
     gbqrf = Symbol(:bh_, p, :gbqrf); gbqrs = Symbol(:bh_, p, :gbqrs)
     tbsv = Symbol(:bh_, p, :tbsv); tbmv = Symbol(:bh_, p, :tbmv)
+    gbscal = Symbol(:bh_, p, :gbscal); gbaxpy = Symbol(:bh_, p, :gbaxpy); gbcopy = Symbol(:bh_, p, :gbcopy)
     @eval begin
+        # lmul!(α, A) on the band data (src/generic/broadcast.jl:942-958)
+        function _gbscal!(α::$T, A::HIPBanded{$T})
+            l, u = bandwidths(A); d = bandeddata(A); m, n = size(A)
+            with_stream() do
+                chk(ccall(($(QuoteNode(gbscal)), libbh), Cint,
+                          (Int64, Int64, Int64, Int64, $T, Ptr{$T}, Int64, Int64, Int64, Int64, Int64, Cint),
+                          m, n, l, u, α, dptr(d), stride(d, 2), 0, m, 0, n, 0), $(QuoteNode(gbscal)))
+            end
+            A
+        end
+        # Y <- a X + Y (banded_generic_axpy!, src/generic/broadcast.jl:978-1020)
+        function _gbaxpy!(a::$T, X::HIPBanded{$T}, Y::HIPBanded{$T})
+            size(X) == size(Y) || throw(DimensionMismatch("X has size $(size(X)) but Y has size $(size(Y))"))
+            (xl, xu), (yl, yu) = bandwidths(X), bandwidths(Y); dx, dy = bandeddata(X), bandeddata(Y)
+            with_stream() do
+                chk(ccall(($(QuoteNode(gbaxpy)), libbh), Cint,
+                          (Int64, Int64, $T, Ptr{$T}, Int64, Int64, Int64, Ptr{$T}, Int64, Int64, Int64),
+                          size(X, 1), size(X, 2), a, dptr(dx), stride(dx, 2), xl, xu, dptr(dy), stride(dy, 2), yl, yu),
+                    $(QuoteNode(gbaxpy)))
+            end
+            Y
+        end
+        # copyto!(B, A) between banded matrices (copyto_bandeddata!, src/banded/BandedMatrix.jl:124-130)
+        function _gbcopy!(B::HIPBanded{$T}, A::HIPBanded{$T})
+            size(A) == size(B) || throw(DimensionMismatch("A has size $(size(A)) but B has size $(size(B))"))
+            (al, au), (bl, bu) = bandwidths(A), bandwidths(B); da, db = bandeddata(A), bandeddata(B)
+            with_stream() do
+                chk(ccall(($(QuoteNode(gbcopy)), libbh), Cint,
+                          (Int64, Int64, Ptr{$T}, Int64, Int64, Int64, Ptr{$T}, Int64, Int64, Int64),
+                          size(A, 1), size(A, 2), dptr(da), stride(da, 2), al, au, dptr(db), stride(db, 2), bl, bu),
+                    $(QuoteNode(gbcopy)))
+            end
+            B
+        end
         # tbsv!/tbmv!(uplo, trans, diag, m, k, bandeddata(Tri(A)), x) (src/blas.jl:71-147): the
         # triangle's band is rows 1:u+1 ('U') or u+1:l+u+1 ('L') of A's band array
         # (src/tribanded.jl:19-26), passed as a pointer into it with A's leading dimension.
@@ -165,6 +202,11 @@ LinearAlgebra.ldiv!(F::BandedLU{T,<:HIPBanded{T}}, B::ROCVecOrMat{T}) where {T} 
 LinearAlgebra.ldiv!(F::Transpose{T,<:BandedLU{T,<:HIPBanded{T}}}, B::ROCVecOrMat{T}) where {T} = _gbtrs!('T', parent(F), B)
 LinearAlgebra.ldiv!(F::Adjoint{T,<:BandedLU{T,<:HIPBanded{T}}}, B::ROCVecOrMat{T}) where {T<:Real} = _gbtrs!('T', parent(F), B)
 materialize!(M::Ldiv{HIPBC}) = ldiv!(lu(M.A), M.B)
+BandedMatrices._banded_lmul!(α::Number, A::HIPBanded{T}, ::HIPBC) where {T} = _gbscal!(T(α), A)
+BandedMatrices._banded_rmul!(A::HIPBanded{T}, α::Number, ::HIPBC) where {T} = _gbscal!(T(α), A)
+BandedMatrices._banded_axpy!(a::Number, X::HIPBanded{T}, Y::HIPBanded{T}, ::HIPBC, ::HIPBC) where {T} =
+    _gbaxpy!(T(a), X, Y)
+BandedMatrices.copyto_bandeddata!(B::HIPBanded{T}, ::HIPBC, A::HIPBanded{T}, ::HIPBC) where {T} = _gbcopy!(B, A)
 # Triangular views: the layouts triangularlayout(Tri, ::BandedColumns) gives
 # (src/tribanded.jl:9-11), with UPLO in ('U','L') and UNIT in ('N','U').
 materialize!(M::Ldiv{<:TriangularLayout{UPLO,UNIT,HIPBC},HIPColumnMajor,<:Any,<:ROCVector}) where {UPLO,UNIT} =
