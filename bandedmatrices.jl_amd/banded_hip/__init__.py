@@ -14,6 +14,7 @@ from .mixed import MixedResult, MixedSolver, solve_mixed
 from .adjoint import band_outer_, loss_and_grad, mul_pullback, solve_pullback
 from .batch import BatchedGBSV, BatchGraph, c4_batch
 from .qr import BandedQR, QRPackedQ, qr, qr_, qr_ldiv_, qr_lmul_, qr_solve
+from .symbanded import BandedCholesky, PosDefException, Symmetric, cholesky, cholesky_ldiv_
 from .tribanded import (BandView, LowerTriangular, UnitLowerTriangular, UnitUpperTriangular, UpperTriangular, tbmv_,
                         tbsv_)
 
@@ -23,5 +24,6 @@ __all__ = [
     "SingularException", "TransposeFact", "ldiv_", "lu", "lu_", "pad_for_lu", "solve", "MixedResult",
     "MixedSolver", "solve_mixed", "BandedQR", "QRPackedQ", "qr", "qr_", "qr_ldiv_", "qr_lmul_", "qr_solve",
     "BatchedGBSV", "BatchGraph", "c4_batch", "band_outer_", "loss_and_grad", "mul_pullback", "solve_pullback",
-    "axpy_", "convert", "copyto_", "BandView", "UpperTriangular", "LowerTriangular", "UnitUpperTriangular", "UnitLowerTriangular", "tbsv_", "tbmv_",
+    "axpy_", "convert", "copyto_", "Symmetric", "BandedCholesky", "PosDefException", "cholesky", "cholesky_ldiv_",
+    "BandView", "UpperTriangular", "LowerTriangular", "UnitUpperTriangular", "UnitLowerTriangular", "tbsv_", "tbmv_",
 ]

@@ -1,0 +1,252 @@
+// bh_pbtrf.hip -- symmetric positive definite band Cholesky, its solve, and
+// the symmetric band product (LAPACK xPBTRF / xPBTRS, BLAS xSBMV) on gfx950.
+//
+// Storage (LAPACK symmetric band): uplo 'U' keeps A(i,j), i <= j, at
+// ab[kd + i - j + j*ldab]; 'L' keeps A(i,j), i >= j, at ab[i - j + j*ldab].
+//
+// Factorisation, restating xPBTF2 step for step (A = U'U or L L'): column j
+// gets a_jj <- sqrt(a_jj) (stop with info = j+1 when a_jj <= 0; a NaN goes on), the kn
+// = min(kd, n-1-j) entries of its row of U (column of L) are multiplied by the
+// rounded reciprocal 1/a_jj, and the trailing kn x kn triangle takes the xSYR
+// rank-1 update with alpha = -1: column q is skipped when v_q == 0, otherwise
+// a_pq = fma(-v_q, v_p, a_pq).  Every entry sees the same operations in the
+// same order as on the host, so factor and info are bit-identical.
+//
+// One workgroup walks the n steps with the kd+1 live columns in LDS (column c
+// in slot c mod (kd+1)); per step: scale the row (one thread per entry), a
+// barrier, the rank-1 update (one thread per column, which for column j+1
+// also takes the next square root), write-out of the finished column j and
+// refill of its slot with column j+kd+1, a barrier.
+#include "bh_common.hpp"
+#include "bh_tbsv.hpp"
+
+namespace bh {
+namespace {
+
+template <typename T>
+__device__ __forceinline__ T dsqrt(T x);
+template <>
+__device__ __forceinline__ double dsqrt<double>(double x) { return __builtin_sqrt(x); }
+template <>
+__device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_sqrtf(x); }
+
+template <typename T, bool UP>
+__global__ void pbtrf_lds_kernel(int n, int kd, T *__restrict__ ab, i64 ldab, i64 *__restrict__ info_out) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int H = kd + 1;
+    T *W = reinterpret_cast<T *>(smem_raw);  // H slots of H entries
+    T *V = W + (i64)H * H;                   // v_1 .. v_kd of the current step
+    __shared__ T sR;                         // 1 / a_jj of the current step
+    __shared__ int sInfo;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int dr = UP ? kd : 0;  // diagonal row
+    auto slot = [&](int c) { return W + (i64)(c % H) * H; };
+    // first window: columns 0 .. min(kd, n-1)
+    for (int e = tid; e < H * H; e += nt) {
+        const int c = e / H, r = e - c * H;
+        if (c < n) W[e] = ab[(i64)c * ldab + r];
+    }
+    if (tid == 0) sInfo = 0;
+    __syncthreads();
+    if (tid == 0) {
+        const T d = W[dr];
+        if (d <= T(0)) {
+            sInfo = 1;
+        } else {
+            const T s = dsqrt(d);
+            W[dr] = s;
+            sR = T(1) / s;
+        }
+    }
+    __syncthreads();
+    int done = 0;  // columns written back
+    for (int j = 0; j < n; ++j) {
+        if (sInfo) break;
+        const int kn = min(kd, n - 1 - j);
+        // scale: v_c = U(j, j+c) (row kd-c of column j+c) or L(j+c, j) (row c of column j)
+        for (int c = 1 + tid; c <= kn; c += nt) {
+            T *pv = UP ? slot(j + c) + (kd - c) : slot(j) + c;
+            const T v = *pv * sR;
+            *pv = v;
+            V[c] = v;
+        }
+        __syncthreads();
+        // rank-1 update, thread per column q of the trailing triangle
+        for (int q = 1 + tid; q <= kn; q += nt) {
+            T *col = slot(j + q);
+            const T vq = V[q];
+            if (vq != T(0)) {
+                const T tq = -vq;
+                if (UP) {
+                    for (int p = 1; p <= q; ++p) col[kd + p - q] = dfma(tq, V[p], col[kd + p - q]);
+                } else {
+                    for (int p = q; p <= kn; ++p) col[p - q] = dfma(tq, V[p], col[p - q]);
+                }
+            }
+            if (q == 1) {  // column j+1's diagonal is final: its square root for the next step
+                const T d = col[dr];
+                if (d <= T(0)) {
+                    sInfo = j + 2;
+                } else {
+                    const T s = dsqrt(d);
+                    col[dr] = s;
+                    sR = T(1) / s;
+                }
+            }
+        }
+        // column j is final: write it out, refill its slot with column j+kd+1
+        {
+            T *sj = slot(j);
+            const int cin = j + H;
+            for (int r = tid; r < H; r += nt) {
+                ab[(i64)j * ldab + r] = sj[r];
+                if (cin < n) sj[r] = ab[(i64)cin * ldab + r];
+            }
+            // kd == 0: no update ran, and column j+1 has just entered slot(j) (thread 0 refilled it)
+            if (kd == 0 && tid == 0 && j + 1 < n) {
+                const T d = sj[0];
+                if (d <= T(0)) {
+                    sInfo = j + 2;
+                } else {
+                    const T s = dsqrt(d);
+                    sj[0] = s;
+                    sR = T(1) / s;
+                }
+            }
+        }
+        done = j + 1;
+        __syncthreads();
+    }
+    // stopped early: the columns still in the window go back as they stand
+    if (done < n) {
+        for (int c = done; c < min(n, done + H); ++c)
+            for (int r = tid; r < H; r += nt) ab[(i64)c * ldab + r] = slot(c)[r];
+    }
+    if (tid == 0) *info_out = sInfo;
+}
+
+template <typename T>
+int pbtrf_impl(char uplo, i64 n, i64 kd, T *ab, i64 ldab, i64 *info_host) {
+    const bool up = (uplo == 'U' || uplo == 'u'), lo = (uplo == 'L' || uplo == 'l');
+    BH_ARG(up || lo, 1);
+    BH_ARG(n >= 0, 2);
+    BH_ARG(kd >= 0, 3);
+    BH_ARG(ldab >= kd + 1, 5);
+    BH_ARG(info_host != nullptr, 6);
+    *info_host = 0;
+    if (n == 0) return BH_OK;
+    const i64 H = kd + 1;
+    const size_t lds = (size_t)(H * H + H) * sizeof(T);
+    if (lds > 150 * 1024 || n >= (1ll << 31) || ldab * n * (i64)sizeof(T) >= (1ll << 62)) {
+        set_error("pbtrf: bandwidth kd = %lld needs %zu bytes of LDS (at most %d)", (long long)kd, lds, 150 * 1024);
+        return -3;
+    }
+    hipStream_t st = stream();
+    i64 *dinfo = nullptr;
+    BH_HIP(hipMallocAsync((void **)&dinfo, sizeof(i64), st));
+    const int nt = (int)imin(256, 64 * cdiv(H, 64));
+    auto kern = up ? pbtrf_lds_kernel<T, true> : pbtrf_lds_kernel<T, false>;
+    hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(kern, dim3(1), dim3(nt), lds, st, (int)n, (int)kd, ab, ldab, dinfo);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(info_host, dinfo, sizeof(i64), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    const hipError_t ef = hipFreeAsync(dinfo, st);
+    if (e != hipSuccess) return hip_fail(e, "pbtrf", __FILE__, __LINE__);
+    BH_HIP(ef);
+    return BH_OK;
+}
+
+// xPBTRS: U'U x = b as U' y = b ('T' upper), U x = y ('N' upper); L L' x = b as
+// L y = b ('N' lower), L' x = y ('T' lower), every column of B in one launch.
+template <typename T>
+int pbtrs_impl(char uplo, i64 n, i64 kd, i64 nrhs, const T *ab, i64 ldab, T *B, i64 ldb) {
+    const bool up = (uplo == 'U' || uplo == 'u'), lo = (uplo == 'L' || uplo == 'l');
+    BH_ARG(up || lo, 1);
+    BH_ARG(n >= 0, 2);
+    BH_ARG(kd >= 0, 3);
+    BH_ARG(nrhs >= 0 && nrhs < (1ll << 31), 4);
+    BH_ARG(ldab >= kd + 1, 6);
+    BH_ARG(ldb >= (n > 1 ? n : 1), 8);
+    BH_ARG(n < (1ll << 30) && kd < (1ll << 30) && ldab * n * (i64)sizeof(T) < (1ll << 31) - (1ll << 24), 2);
+    if (n == 0 || nrhs == 0) return BH_OK;
+    const int ni = (int)n, ki = (int)kd, nr = (int)nrhs;
+    int st = tbsv_run(up ? 2 : 1, ni, ki, ab, ldab, B, 1, ldb, nr, false);
+    if (st != BH_OK) return st;
+    return tbsv_run(up ? 0 : 3, ni, ki, ab, ldab, B, 1, ldb, nr, false);
+}
+
+// ---- xSBMV: y <- alpha*A*x + beta*y, one thread per row; row i sums
+// fma(A(i,j), x_j, t) for j = i-k .. i+k ascending from t = 0, then
+// y_i = alpha*t (beta == 0) or alpha*t + beta*y_i.
+template <typename T, bool UP>
+__global__ void __launch_bounds__(256) sbmv_rows(int n, int k, T alpha, const T *__restrict__ ab, i64 ldab,
+                                                 const T *__restrict__ x, int incx, T beta, T *__restrict__ y,
+                                                 int incy) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    T t = T(0);
+    const int j0 = max(0, i - k), j1 = min(n - 1, i + k);
+    for (int j = j0; j <= j1; ++j) {
+        const int r = min(i, j), c = max(i, j);  // the stored entry (r, c), r <= c
+        const T a = UP ? ab[(i64)c * ldab + (k + r - c)] : ab[(i64)r * ldab + (c - r)];
+        t = dfma(a, x[(i64)j * incx], t);
+    }
+    T *py = y + (i64)i * incy;
+    *py = (beta == T(0)) ? alpha * t : alpha * t + beta * *py;
+}
+
+template <typename T>
+int sbmv_impl(char uplo, i64 n, i64 k, T alpha, const T *ab, i64 ldab, const T *x, i64 incx, T beta, T *y,
+              i64 incy) {
+    const bool up = (uplo == 'U' || uplo == 'u'), lo = (uplo == 'L' || uplo == 'l');
+    BH_ARG(up || lo, 1);
+    BH_ARG(n >= 0, 2);
+    BH_ARG(k >= 0, 3);
+    BH_ARG(ldab >= k + 1, 6);
+    BH_ARG(incx >= 1, 8);
+    BH_ARG(incy >= 1, 11);
+    BH_ARG(n < (1ll << 30) && k < (1ll << 30) && n * incx < (1ll << 31) && n * incy < (1ll << 31), 2);
+    if (n == 0) return BH_OK;
+    const int nb = (int)cdiv(n, 256);
+    if (up)
+        hipLaunchKernelGGL((sbmv_rows<T, true>), dim3(nb), dim3(256), 0, stream(), (int)n, (int)k, alpha, ab, ldab, x,
+                           (int)incx, beta, y, (int)incy);
+    else
+        hipLaunchKernelGGL((sbmv_rows<T, false>), dim3(nb), dim3(256), 0, stream(), (int)n, (int)k, alpha, ab, ldab,
+                           x, (int)incx, beta, y, (int)incy);
+    BH_LAUNCH_CHECK();
+    return BH_OK;
+}
+
+}  // namespace
+}  // namespace bh
+
+using namespace bh;
+
+extern "C" {
+
+int bh_dpbtrf(char uplo, bh_int n, bh_int kd, double *AB, bh_int ldab, bh_int *info_host) {
+    return pbtrf_impl<double>(uplo, n, kd, AB, ldab, info_host);
+}
+int bh_spbtrf(char uplo, bh_int n, bh_int kd, float *AB, bh_int ldab, bh_int *info_host) {
+    return pbtrf_impl<float>(uplo, n, kd, AB, ldab, info_host);
+}
+int bh_dpbtrs(char uplo, bh_int n, bh_int kd, bh_int nrhs, const double *AB, bh_int ldab, double *B, bh_int ldb) {
+    return pbtrs_impl<double>(uplo, n, kd, nrhs, AB, ldab, B, ldb);
+}
+int bh_spbtrs(char uplo, bh_int n, bh_int kd, bh_int nrhs, const float *AB, bh_int ldab, float *B, bh_int ldb) {
+    return pbtrs_impl<float>(uplo, n, kd, nrhs, AB, ldab, B, ldb);
+}
+int bh_dsbmv(char uplo, bh_int n, bh_int k, double alpha, const double *A, bh_int lda, const double *x, bh_int incx,
+             double beta, double *y, bh_int incy) {
+    return sbmv_impl<double>(uplo, n, k, alpha, A, lda, x, incx, beta, y, incy);
+}
+int bh_ssbmv(char uplo, bh_int n, bh_int k, float alpha, const float *A, bh_int lda, const float *x, bh_int incx,
+             float beta, float *y, bh_int incy) {
+    return sbmv_impl<float>(uplo, n, k, alpha, A, lda, x, incx, beta, y, incy);
+}
+
+}  // extern "C"

@@ -162,6 +162,35 @@ int bh_dgbqrs(bh_int m, bh_int n, bh_int l, bh_int u, const double *D, bh_int ld
 int bh_sgbqrs(bh_int m, bh_int n, bh_int l, bh_int u, const float *D, bh_int ldd, const float *tau,
               bh_int nrhs, float *B, bh_int ldb);
 
+/* ----------------------------------------------------------- symmetric */
+/* Symmetric positive definite band Cholesky A = U'U (uplo 'U') or L L' ('L')
+ * in LAPACK symmetric band storage (U: A(i,j), i <= j, at AB[kd+i-j + j*ldab];
+ * L: A(i,j), i >= j, at AB[i-j + j*ldab]), in place, with LAPACK's info
+ * (> 0: the leading minor of that order is not positive definite; the
+ * columns up to it are factored and the rest left as the update left them).
+ * Bit-identical to xPBTF2.  kd up to 137 (FP64) / 195 (FP32): the live
+ * columns sit in LDS.  Replaces pbtrf! (src/lapack.jl:268-292) as
+ * cholesky(Symmetric(A)) calls it (src/symbanded/BandedCholesky.jl:1-11). */
+int bh_dpbtrf(char uplo, bh_int n, bh_int kd, double *AB, bh_int ldab, bh_int *info_host);
+int bh_spbtrf(char uplo, bh_int n, bh_int kd, float *AB, bh_int ldab, bh_int *info_host);
+/* Solve A X = B with the factor of bh_?pbtrf: two triangular band solves per
+ * column (U' then U, or L then L'), all columns in one launch each.
+ * Replaces pbtrs! (src/lapack.jl:297-333) as ldiv!(::Cholesky, B) calls it
+ * (src/symbanded/BandedCholesky.jl:66-74). */
+int bh_dpbtrs(char uplo, bh_int n, bh_int kd, bh_int nrhs, const double *AB, bh_int ldab, double *B,
+              bh_int ldb);
+int bh_spbtrs(char uplo, bh_int n, bh_int kd, bh_int nrhs, const float *AB, bh_int ldab, float *B,
+              bh_int ldb);
+/* y <- alpha*A*x + beta*y for the symmetric band A stored as one triangle (as
+ * above, k off-diagonals); y not read when beta == 0.  Row i sums its terms
+ * in column order with one fma each.  Replaces BLAS.sbmv! as
+ * materialize!(::MulAdd{SymmetricLayout{BandedColumns}}) calls it
+ * (src/symbanded/symbanded.jl:72-91). */
+int bh_dsbmv(char uplo, bh_int n, bh_int k, double alpha, const double *A, bh_int lda,
+             const double *x, bh_int incx, double beta, double *y, bh_int incy);
+int bh_ssbmv(char uplo, bh_int n, bh_int k, float alpha, const float *A, bh_int lda,
+             const float *x, bh_int incx, float beta, float *y, bh_int incy);
+
 /* -------------------------------------------------------- element-wise */
 /* Scale the stored entries A(i,j), r0 <= i < r1, c0 <= j < c1 (0-based,
  * half-open), of an m x n band (kl, ku, either may be negative with

@@ -14,6 +14,7 @@
  *   - LAPACK ?gbtrs 'N'/'T'         (src/banded/linalg.jl:24-47)
  *   - BLAS ?tbsv upper              (src/tribanded.jl:75-83, src/blas.jl:123-141)
  *   - BLAS ?tbsv / ?tbmv, all cases (src/tribanded.jl:47-54,75-96, src/blas.jl:71-147)
+ *   - LAPACK ?pbtf2 and BLAS ?sbmv  (src/symbanded/BandedCholesky.jl:1-11, symbanded.jl:72-91)
  *   - banded Householder QR         (src/banded/bandedqr.jl:8-35) with the
  *     reflector!/reflectorApply! pair it imports from ArrayLayouts
  *   - Q / Q' application and solve  (src/banded/bandedqr.jl:38-94,175-206)
@@ -346,6 +347,78 @@ void bo_dtbmv(char uplo, char trans, char diag, i64 n, i64 k, const double *a, i
 }
 void bo_stbmv(char uplo, char trans, char diag, i64 n, i64 k, const float *a, i64 lda, float *x, i64 incx) {
     TBMV_GEN_BODY(float, fmaf)
+}
+
+/* ------------------------------------------------------- symmetric */
+/* Band Cholesky A = U'U ('U') or L L' ('L') in LAPACK's symmetric band storage
+ * (uplo 'U': A(i,j), i <= j, at ab[kd + i - j + j*ldab]; 'L': A(i,j), i >= j,
+ * at ab[i - j + j*ldab]), restating xPBTF2, which xPBTRF runs for kd < 32
+ * (src/symbanded/BandedCholesky.jl:1-11, src/lapack.jl:268-292): for each
+ * column j, a_jj <- sqrt(a_jj) (info = j+1 when a_jj <= 0; NaN goes on), the kn
+ * off-diagonal entries of its row (U) or column (L) times the rounded
+ * reciprocal 1/a_jj, then the trailing kn x kn triangle updated as xSYR does
+ * with alpha = -1: column q skipped when v_q == 0, else
+ * a_pq = fma(-v_q, v_p, a_pq). */
+#define PBTF2_BODY(T, SQRT, FMA)                                                        \
+    const int up = (uplo == 'U' || uplo == 'u');                                        \
+    for (i64 j = 0; j < n; ++j) {                                                       \
+        T *dj = &ab[IDX(ldab, up ? kd : 0, j)];                                         \
+        T ajj = *dj;                                                                    \
+        if (ajj <= 0) return j + 1; /* NaN passes, as in LAPACK */                      \
+        ajj = SQRT(ajj);                                                                \
+        *dj = ajj;                                                                      \
+        const i64 kn = imin(kd, n - 1 - j);                                             \
+        if (kn == 0) continue;                                                          \
+        const T r = (T)1 / ajj;                                                         \
+        /* v_c: U(j, j+c) at row kd-c of column j+c, or L(j+c, j) at row c of column j */ \
+        for (i64 c = 1; c <= kn; ++c) {                                                 \
+            T *v = up ? &ab[IDX(ldab, kd - c, j + c)] : &ab[IDX(ldab, c, j)];           \
+            *v = *v * r;                                                                \
+        }                                                                               \
+        for (i64 q = 1; q <= kn; ++q) {                                                 \
+            const T vq = up ? ab[IDX(ldab, kd - q, j + q)] : ab[IDX(ldab, q, j)];       \
+            if (vq == 0) continue;                                                      \
+            const T tq = -vq;                                                           \
+            if (up) { /* column j+q, rows j+1 .. j+q: A(j+p, j+q) at row kd + p - q */  \
+                for (i64 p = 1; p <= q; ++p)                                            \
+                    ab[IDX(ldab, kd + p - q, j + q)] =                                  \
+                        FMA(tq, ab[IDX(ldab, kd - p, j + p)], ab[IDX(ldab, kd + p - q, j + q)]); \
+            } else { /* column j+q, rows j+q .. j+kn: A(j+p, j+q) at row p - q */       \
+                for (i64 p = q; p <= kn; ++p)                                           \
+                    ab[IDX(ldab, p - q, j + q)] =                                       \
+                        FMA(tq, ab[IDX(ldab, p, j)], ab[IDX(ldab, p - q, j + q)]);      \
+            }                                                                           \
+        }                                                                               \
+    }                                                                                   \
+    return 0;
+
+i64 bo_dpbtf2(char uplo, i64 n, i64 kd, double *ab, i64 ldab) { PBTF2_BODY(double, sqrt, fma) }
+i64 bo_spbtf2(char uplo, i64 n, i64 kd, float *ab, i64 ldab) { PBTF2_BODY(float, sqrtf, fmaf) }
+
+/* Symmetric band product y <- alpha*A*x + beta*y (BLAS xSBMV, as
+ * src/symbanded/symbanded.jl:72-91 calls it), A stored as one triangle like
+ * the Cholesky input.  Row i: t = sum over j = i-k .. i+k ascending of
+ * fma(A(i,j), x_j, t) from t = 0; y_i = alpha*t (beta == 0) or
+ * alpha*t + beta*y_i. */
+#define SBMV_BODY(T, FMA)                                                               \
+    const int up = (uplo == 'U' || uplo == 'u');                                        \
+    for (i64 i = 0; i < n; ++i) {                                                       \
+        T t = 0;                                                                        \
+        for (i64 j = imax(0, i - k); j <= imin(n - 1, i + k); ++j) {                    \
+            const i64 r = imin(i, j), c = imax(i, j); /* stored entry (r, c), r <= c */ \
+            const T a = up ? ab[IDX(ldab, k + r - c, c)] : ab[IDX(ldab, c - r, r)];     \
+            t = FMA(a, x[j], t);                                                        \
+        }                                                                               \
+        y[i] = (beta == 0) ? alpha * t : alpha * t + beta * y[i];                       \
+    }
+
+void bo_dsbmv(char uplo, i64 n, i64 k, double alpha, const double *ab, i64 ldab, const double *x, double beta,
+              double *y) {
+    SBMV_BODY(double, fma)
+}
+void bo_ssbmv(char uplo, i64 n, i64 k, float alpha, const float *ab, i64 ldab, const float *x, float beta,
+              float *y) {
+    SBMV_BODY(float, fmaf)
 }
 
 /* ------------------------------------------------------------- gbtrs */

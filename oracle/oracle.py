@@ -60,6 +60,12 @@ def lib():
                 f = getattr(L, f"bo_{pre}{nm}")
                 f.argtypes = [C.c_char, C.c_char, C.c_char, i64, i64, fp, i64, fp, i64]
                 f.restype = None
+            f = getattr(L, f"bo_{pre}pbtf2")
+            f.argtypes = [C.c_char, i64, i64, fp, i64]
+            f.restype = i64
+            f = getattr(L, f"bo_{pre}sbmv")
+            f.argtypes = [C.c_char, i64, i64, ft, fp, i64, fp, ft, fp]
+            f.restype = None
         L.bo_dgbtf2_planted.argtypes = [C.c_int, i64, i64, i64, i64, _dp, i64, _ip]
         L.bo_dgbtf2_planted.restype = i64
         L.bo_dgbmm.argtypes = [i64, i64, i64, C.c_double, _dp, i64, i64, i64, _dp, i64, i64, i64,
@@ -173,6 +179,33 @@ def tbmv(uplo, trans, diag, n, k, a, x):
     getattr(lib(), f"bo_{pre}tbmv")(uplo.encode(), trans.encode(), diag.encode(), n, k, _ptr(aa, P), aa.shape[0],
                                     _ptr(xx, P), 1)
     return xx
+
+
+def pbtf2(uplo, n, kd, ab):
+    """Band Cholesky of the symmetric band ab ((kd+1) x n, LAPACK storage): (factor, info)."""
+    pre, P = _pre(ab.dtype)
+    f = _f(ab).copy(order="F")
+    info = getattr(lib(), f"bo_{pre}pbtf2")(uplo.encode(), n, kd, _ptr(f, P), f.shape[0])
+    return f, int(info)
+
+
+def pbtrs(uplo, n, kd, f, b):
+    """Solve with the band Cholesky factor: two triangular band solves, as xPBTRS does."""
+    if uplo == "U":
+        y = tbsv("U", "T", "N", n, kd, f, b)
+        return tbsv("U", "N", "N", n, kd, f, y)
+    y = tbsv("L", "N", "N", n, kd, f, b)
+    return tbsv("L", "T", "N", n, kd, f, y)
+
+
+def sbmv(uplo, n, k, alpha, ab, x, beta=0.0, y=None):
+    pre, P = _pre(ab.dtype)
+    aa = _f(ab)
+    xx = np.ascontiguousarray(x, dtype=ab.dtype)
+    yy = np.zeros(n, ab.dtype) if y is None else np.ascontiguousarray(y, dtype=ab.dtype).copy()
+    getattr(lib(), f"bo_{pre}sbmv")(uplo.encode(), n, k, alpha, _ptr(aa, P), aa.shape[0], _ptr(xx, P), beta,
+                                    _ptr(yy, P))
+    return yy
 
 
 def gbqrf(m, n, l, u, D):
