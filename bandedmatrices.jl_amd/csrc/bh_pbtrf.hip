@@ -16,7 +16,10 @@
 // in slot c mod (kd+1)); per step: scale the row (one thread per entry), a
 // barrier, the rank-1 update (one thread per column, which for column j+1
 // also takes the next square root), write-out of the finished column j and
-// refill of its slot with column j+kd+1, a barrier.
+// refill of its slot with column j+kd+1, a barrier.  The barriers order LDS
+// only.  Incoming columns arrive in chunks of B: a chunk is loaded into
+// registers while the previous one is consumed and stored to LDS when it is
+// done, so no step waits on HBM.
 #include "bh_common.hpp"
 #include "bh_tbsv.hpp"
 
@@ -30,92 +33,112 @@ __device__ __forceinline__ double dsqrt<double>(double x) { return __builtin_sqr
 template <>
 __device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_sqrtf(x); }
 
+constexpr int kPbRegs = 24;  // staged values per thread: chunk columns x (kd+1) <= kPbRegs x threads
+
 template <typename T, bool UP>
-__global__ void pbtrf_lds_kernel(int n, int kd, T *__restrict__ ab, i64 ldab, i64 *__restrict__ info_out) {
+__global__ void __launch_bounds__(256) pbtrf_lds_kernel(int n, int kd, T *__restrict__ ab, i64 ldab, i64 *__restrict__ info_out, int B) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int H = kd + 1;
-    T *W = reinterpret_cast<T *>(smem_raw);  // H slots of H entries
-    T *V = W + (i64)H * H;                   // v_1 .. v_kd of the current step
+    // slot stride: even, so that entries one row apart in neighbouring slots
+    // (what neighbouring threads touch) sit an odd number of doubles apart and
+    // fall in different LDS banks
+    const int SL = H + (H & 1);
+    T *W = reinterpret_cast<T *>(smem_raw);  // H slots of SL entries: the live columns
+    T *V = W + (i64)H * SL;                  // v_1 .. v_kd of the current step
+    T *S = V + H;                            // the next B incoming columns (column j+H at S[(j mod B)*H])
     __shared__ T sR;                         // 1 / a_jj of the current step
     __shared__ int sInfo;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int dr = UP ? kd : 0;  // diagonal row
-    auto slot = [&](int c) { return W + (i64)(c % H) * H; };
-    // first window: columns 0 .. min(kd, n-1)
+    const int BH = B * H;
+    auto slot = [&](int c) { return W + (i64)(c % H) * SL; };
+    // first window (columns 0 .. kd) and first chunk (columns H .. H+B-1)
     for (int e = tid; e < H * H; e += nt) {
         const int c = e / H, r = e - c * H;
-        if (c < n) W[e] = ab[(i64)c * ldab + r];
+        if (c < n) W[(i64)c * SL + r] = ab[(i64)c * ldab + r];
+    }
+    for (int e = tid; e < BH; e += nt) {
+        const int c = H + e / H, r = e % H;
+        S[e] = c < n ? ab[(i64)c * ldab + r] : T(0);
     }
     if (tid == 0) sInfo = 0;
     __syncthreads();
-    if (tid == 0) {
-        const T d = W[dr];
-        if (d <= T(0)) {
-            sInfo = 1;
+    auto take_sqrt = [&](T *d, int info_if_bad) {
+        const T v = *d;
+        if (v <= T(0)) {
+            sInfo = info_if_bad;
         } else {
-            const T s = dsqrt(d);
-            W[dr] = s;
+            const T s = dsqrt(v);
+            *d = s;
             sR = T(1) / s;
         }
-    }
-    __syncthreads();
+    };
+    if (tid == 0) take_sqrt(W + dr, 1);  // column 0 sits in slot 0
+    lds_barrier();
     int done = 0;  // columns written back
-    for (int j = 0; j < n; ++j) {
-        if (sInfo) break;
-        const int kn = min(kd, n - 1 - j);
-        // scale: v_c = U(j, j+c) (row kd-c of column j+c) or L(j+c, j) (row c of column j)
-        for (int c = 1 + tid; c <= kn; c += nt) {
-            T *pv = UP ? slot(j + c) + (kd - c) : slot(j) + c;
-            const T v = *pv * sR;
-            *pv = v;
-            V[c] = v;
+    for (int j0 = 0; j0 < n && !sInfo; j0 += B) {
+        // the chunk after this one, loaded now and stored into S when this chunk
+        // is done: the registers are not touched in between, so no step waits
+        T reg[kPbRegs];
+#pragma unroll
+        for (int i = 0; i < kPbRegs; ++i) {
+            const int e = tid + i * nt;
+            const int c = H + j0 + B + e / H, r = e % H;
+            reg[i] = (e < BH && c < n) ? ab[(i64)c * ldab + r] : T(0);
         }
-        __syncthreads();
-        // rank-1 update, thread per column q of the trailing triangle
-        for (int q = 1 + tid; q <= kn; q += nt) {
-            T *col = slot(j + q);
-            const T vq = V[q];
-            if (vq != T(0)) {
-                const T tq = -vq;
-                if (UP) {
-                    for (int p = 1; p <= q; ++p) col[kd + p - q] = dfma(tq, V[p], col[kd + p - q]);
-                } else {
-                    for (int p = q; p <= kn; ++p) col[p - q] = dfma(tq, V[p], col[p - q]);
-                }
+        const int jend = min(n, j0 + B);
+        for (int j = j0; j < jend; ++j) {
+            if (sInfo) break;  // uniform: read after a barrier
+            const int kn = min(kd, n - 1 - j);
+            // scale: v_c = U(j, j+c) (row kd-c of column j+c) or L(j+c, j) (row c of column j)
+            for (int c = 1 + tid; c <= kn; c += nt) {
+                T *pv = UP ? slot(j + c) + (kd - c) : slot(j) + c;
+                const T v = *pv * sR;
+                *pv = v;
+                V[c] = v;
             }
-            if (q == 1) {  // column j+1's diagonal is final: its square root for the next step
-                const T d = col[dr];
-                if (d <= T(0)) {
-                    sInfo = j + 2;
-                } else {
-                    const T s = dsqrt(d);
-                    col[dr] = s;
-                    sR = T(1) / s;
+            lds_barrier();
+            // rank-1 update, thread per column q of the trailing triangle, in
+            // chunks of 8 entries whose reads are issued before their writes
+            for (int q = 1 + tid; q <= kn; q += nt) {
+                T *col = slot(j + q);
+                const T vq = V[q];
+                if (vq != T(0)) {
+                    const T tq = -vq;
+                    const int p0 = UP ? 1 : q, p1 = UP ? q : kn;  // p0 .. p1 inclusive
+                    T *a0 = UP ? col + (kd - q) : col - q;      // A(j+p, j+q) at a0[p]
+                    int p = p0;
+                    for (; p + 8 <= p1 + 1; p += 8) {
+                        T vv[8], aa[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            vv[u] = V[p + u];
+                            aa[u] = a0[p + u];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) a0[p + u] = dfma(tq, vv[u], aa[u]);
+                    }
+                    for (; p <= p1; ++p) a0[p] = dfma(tq, V[p], a0[p]);
                 }
+                if (q == 1) take_sqrt(col + dr, j + 2);  // column j+1's diagonal is final
             }
-        }
-        // column j is final: write it out, refill its slot with column j+kd+1
-        {
+            // column j is final: write it out; its slot takes column j+kd+1 from S
             T *sj = slot(j);
-            const int cin = j + H;
-            for (int r = tid; r < H; r += nt) {
-                ab[(i64)j * ldab + r] = sj[r];
-                if (cin < n) sj[r] = ab[(i64)cin * ldab + r];
+            if (tid < H) {
+                ab[(i64)j * ldab + tid] = sj[tid];
+                if (j + H < n) sj[tid] = S[(j - j0) * H + tid];
             }
-            // kd == 0: no update ran, and column j+1 has just entered slot(j) (thread 0 refilled it)
-            if (kd == 0 && tid == 0 && j + 1 < n) {
-                const T d = sj[0];
-                if (d <= T(0)) {
-                    sInfo = j + 2;
-                } else {
-                    const T s = dsqrt(d);
-                    sj[0] = s;
-                    sR = T(1) / s;
-                }
-            }
+            // kd == 0: no update ran, and column j+1 has just entered slot(j)
+            if (kd == 0 && tid == 0 && j + 1 < n) take_sqrt(sj, j + 2);
+            done = j + 1;
+            lds_barrier();
         }
-        done = j + 1;
-        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kPbRegs; ++i) {
+            const int e = tid + i * nt;
+            if (e < BH) S[e] = reg[i];
+        }
+        lds_barrier();
     }
     // stopped early: the columns still in the window go back as they stand
     if (done < n) {
@@ -136,19 +159,25 @@ int pbtrf_impl(char uplo, i64 n, i64 kd, T *ab, i64 ldab, i64 *info_host) {
     *info_host = 0;
     if (n == 0) return BH_OK;
     const i64 H = kd + 1;
-    const size_t lds = (size_t)(H * H + H) * sizeof(T);
-    if (lds > 150 * 1024 || n >= (1ll << 31) || ldab * n * (i64)sizeof(T) >= (1ll << 62)) {
-        set_error("pbtrf: bandwidth kd = %lld needs %zu bytes of LDS (at most %d)", (long long)kd, lds, 150 * 1024);
+    const int nt = (int)imin(256, 64 * cdiv(H, 64));
+    // LDS: the live window, v, and a chunk of B incoming columns (B as large as
+    // the registers and 150 KiB allow, at most 32)
+    const i64 fixed = (H * (H + (H & 1)) + H) * (i64)sizeof(T);
+    const i64 budget = 150 * 1024;
+    const i64 B = imin(imin(32, (i64)kPbRegs * nt / H), (budget - fixed) / (H * (i64)sizeof(T)));
+    if (B < 4 || n >= (1ll << 31)) {
+        set_error("pbtrf: bandwidth kd = %lld does not fit the LDS window (about 130 for FP64, 185 for FP32)",
+                  (long long)kd);
         return -3;
     }
+    const size_t lds = (size_t)(fixed + B * H * (i64)sizeof(T));
     hipStream_t st = stream();
     i64 *dinfo = nullptr;
     BH_HIP(hipMallocAsync((void **)&dinfo, sizeof(i64), st));
-    const int nt = (int)imin(256, 64 * cdiv(H, 64));
     auto kern = up ? pbtrf_lds_kernel<T, true> : pbtrf_lds_kernel<T, false>;
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(kern, dim3(1), dim3(nt), lds, st, (int)n, (int)kd, ab, ldab, dinfo);
+        hipLaunchKernelGGL(kern, dim3(1), dim3(nt), lds, st, (int)n, (int)kd, ab, ldab, dinfo, (int)B);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipMemcpyAsync(info_host, dinfo, sizeof(i64), hipMemcpyDeviceToHost, st);

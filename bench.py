@@ -469,6 +469,65 @@ def bench_elementwise(bh, steps, skip_cpu):
     return r
 
 
+def bench_cholesky(bh, steps, skip_cpu):
+    """Symmetric positive definite band Cholesky (pbtrf), its solve (pbtrs) and
+    the symmetric product (sbmv), FP64, upper storage, n = 65536 at kd = 8 and
+    kd = 128."""
+    import ctypes as C
+
+    import scipy.linalg.blas as sb
+    import scipy.linalg.lapack as la
+
+    out = {}
+    for n, kd in ((65536, 8), (65536, 128)):
+        rng = np.random.default_rng(3)
+        a = np.asfortranarray(rng.uniform(-1.0, 1.0, size=(kd + 1, n)))
+        a[kd, :] = kd + 1 + rng.random(n)
+        b = rng.standard_normal(n)
+        A = bh.DeviceArray.from_numpy(a)
+        info = C.c_int64()
+        f_tr = lambda: (A.upload(a), bh._lib.call("bh_dpbtrf", b"U", n, kd, A.ptr, A.ld, C.byref(info)))
+        f_tr()
+        r = {"n": n, "kd": kd}
+        t = bh.Timer()
+        ms = []
+        for _ in range(max(2, steps // 2)):
+            A.upload(a)
+            t.start()
+            bh._lib.call("bh_dpbtrf", b"U", n, kd, A.ptr, A.ld, C.byref(info))
+            t.stop()
+            ms.append(t.elapsed_ms())
+        r["ms_pbtrf"] = float(np.median(ms))
+        X = bh.DeviceArray.from_numpy(b)
+        f_rs = lambda: bh._lib.call("bh_dpbtrs", b"U", n, kd, 1, A.ptr, A.ld, X.ptr, n)
+        f_rs()
+        r["ms_pbtrs"] = gpu_time(bh, f_rs, max(2, steps // 2))
+        Y = bh.DeviceArray((n,), np.float64)
+        f_mv = lambda: bh._lib.call("bh_dsbmv", b"U", n, kd, 1.0, A.ptr, A.ld, X.ptr, 1, 0.0, Y.ptr, 1)
+        f_mv()
+        r["us_sbmv"] = gpu_time(bh, f_mv, 20) * 1e3
+        r["sbmv_GBps_algorithmic"] = ((kd + 1) * n + 2 * n) * 8 / (r["us_sbmv"] * 1e-6) / 1e9
+        if not skip_cpu:
+            from threadpoolctl import threadpool_limits
+
+            with threadpool_limits(1):
+                t0 = time.perf_counter()
+                for _ in range(3):
+                    c, _ = la.dpbtrf(a)
+                r["cpu_ms_pbtrf_1core"] = (time.perf_counter() - t0) / 3 * 1e3
+                t0 = time.perf_counter()
+                for _ in range(3):
+                    la.dpbtrs(c, b)
+                r["cpu_ms_pbtrs_1core"] = (time.perf_counter() - t0) / 3 * 1e3
+                t0 = time.perf_counter()
+                for _ in range(10):
+                    sb.dsbmv(kd, 1.0, a, b)
+                r["cpu_us_sbmv_1core"] = (time.perf_counter() - t0) / 10 * 1e6
+            r["cpu_sample"] = "OpenBLAS 0.3.28 (scipy) dpbtrf / dpbtrs / dsbmv, upper, 1 thread"
+        out[f"cholesky_n{n}_kd{kd}"] = r
+    return out
+
+
 def traffic_probe(bh, name):
     """A few launches of one kernel plus a 256 MiB device copy, for counter runs."""
     import ctypes as C
@@ -507,7 +566,7 @@ def main():
     ap.add_argument("--headline-only", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--traffic-probe", default=None)
-    ap.add_argument("--only", choices=["triangular", "elementwise"], default=None,
+    ap.add_argument("--only", choices=["triangular", "elementwise", "cholesky"], default=None,
                     help="run one side workload and print its figures as one JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -525,6 +584,9 @@ def main():
         return
     if args.only == "triangular":
         print(json.dumps(bench_triangular(bh, args.steps, args.skip_cpu)), flush=True)
+        return
+    if args.only == "cholesky":
+        print(json.dumps(bench_cholesky(bh, args.steps, args.skip_cpu)), flush=True)
         return
     if args.only == "elementwise":
         print(json.dumps(bench_elementwise(bh, args.steps, args.skip_cpu)), flush=True)
@@ -547,6 +609,8 @@ def main():
         extra.update(bench_triangular(bh, args.steps, args.skip_cpu))
         log("element-wise band operations")
         extra["elementwise_n2p20_8x8"] = bench_elementwise(bh, args.steps, args.skip_cpu)
+        log("symmetric band Cholesky and product")
+        extra.update(bench_cholesky(bh, args.steps, args.skip_cpu))
         log("precision workload, no diagonal shift (pivoting active)")
         P2 = bench_precision(bh, max(1, args.steps // 2), 1, shift=False)
         ns = {"mixed_solves_per_s": 1000.0 / P2["ms_mixed"], "fp64_gpu_solves_per_s": 1000.0 / P2["ms_fp64"],
