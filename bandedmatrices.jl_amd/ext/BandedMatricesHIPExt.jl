@@ -16,6 +16,8 @@
 #   Tri(A) \ x, Tri(A) * x tbsv! / tbmv! on the triangle's band rows  src/tribanded.jl:19-26,47-54,75-96
 #   lmul!/rmul!, axpy!     _banded_lmul!/_rmul!, _banded_axpy!         src/generic/broadcast.jl:942-1020
 #   copyto!(B, A) banded   copyto_bandeddata!                          src/banded/BandedMatrix.jl:124-130
+#   Symmetric(A) * x       BLAS.sbmv!                                  src/symbanded/symbanded.jl:72-91
+#   cholesky / ldiv!       pbtrf! / pbtrs!                             src/symbanded/BandedCholesky.jl:1-79
 module BandedMatricesHIPExt
 
 using BandedMatrices, LinearAlgebra, ArrayLayouts, AMDGPU
@@ -66,7 +68,40 @@ for (T, p) in ((Float64, :d), (Float32, :s))
     gbqrf = Symbol(:bh_, p, :gbqrf); gbqrs = Symbol(:bh_, p, :gbqrs)
     tbsv = Symbol(:bh_, p, :tbsv); tbmv = Symbol(:bh_, p, :tbmv)
     gbscal = Symbol(:bh_, p, :gbscal); gbaxpy = Symbol(:bh_, p, :gbaxpy); gbcopy = Symbol(:bh_, p, :gbcopy)
+    pbtrf = Symbol(:bh_, p, :pbtrf); pbtrs = Symbol(:bh_, p, :pbtrs); sbmv = Symbol(:bh_, p, :sbmv)
     @eval begin
+        # pbtrf!(uplo, n, kd, AB) on the triangle's band rows (src/lapack.jl:268-292)
+        function _pbtrf!(uplo::Char, n::Int, kd::Int, AB::ROCMatrix{$T}, off::Int)
+            info = Ref{Int64}(0)
+            with_stream() do
+                chk(ccall(($(QuoteNode(pbtrf)), libbh), Cint, (UInt8, Int64, Int64, Ptr{$T}, Int64, Ref{Int64}),
+                          uplo, n, kd, dptr(AB) + off * sizeof($T), stride(AB, 2), info), $(QuoteNode(pbtrf)))
+            end
+            info[]
+        end
+        # pbtrs!(uplo, n, kd, AB, B) (src/lapack.jl:297-333)
+        function _pbtrs!(uplo::Char, n::Int, kd::Int, AB::ROCMatrix{$T}, off::Int, B::ROCVecOrMat{$T})
+            ldb = B isa ROCVector ? length(B) : stride(B, 2)
+            with_stream() do
+                chk(ccall(($(QuoteNode(pbtrs)), libbh), Cint,
+                          (UInt8, Int64, Int64, Int64, Ptr{$T}, Int64, Ptr{$T}, Int64),
+                          uplo, n, kd, size(B, 2), dptr(AB) + off * sizeof($T), stride(AB, 2), dptr(B), ldb),
+                    $(QuoteNode(pbtrs)))
+            end
+            B
+        end
+        # y <- α Symmetric(A) x + β y (BLAS.sbmv! on symbandeddata, src/symbanded/symbanded.jl:72-91)
+        function _sbmv!(uplo::Char, α::$T, A::HIPBanded{$T}, x::ROCVector{$T}, β::$T, y::ROCVector{$T})
+            l, u = bandwidths(A); d = bandeddata(A); n = size(A, 1)
+            k, off = uplo == 'U' ? (u, 0) : (l, u)
+            with_stream() do
+                chk(ccall(($(QuoteNode(sbmv)), libbh), Cint,
+                          (UInt8, Int64, Int64, $T, Ptr{$T}, Int64, Ptr{$T}, Int64, $T, Ptr{$T}, Int64),
+                          uplo, n, k, α, dptr(d) + off * sizeof($T), stride(d, 2), dptr(x), 1, β, dptr(y), 1),
+                    $(QuoteNode(sbmv)))
+            end
+            y
+        end
         # lmul!(α, A) on the band data (src/generic/broadcast.jl:942-958)
         function _gbscal!(α::$T, A::HIPBanded{$T})
             l, u = bandwidths(A); d = bandeddata(A); m, n = size(A)
@@ -207,6 +242,22 @@ BandedMatrices._banded_rmul!(A::HIPBanded{T}, α::Number, ::HIPBC) where {T} = _
 BandedMatrices._banded_axpy!(a::Number, X::HIPBanded{T}, Y::HIPBanded{T}, ::HIPBC, ::HIPBC) where {T} =
     _gbaxpy!(T(a), X, Y)
 BandedMatrices.copyto_bandeddata!(B::HIPBanded{T}, ::HIPBC, A::HIPBanded{T}, ::HIPBC) where {T} = _gbcopy!(B, A)
+# Symmetric views: sbmv for the product; banded_chol! / _ldiv! for the factorisation
+# (src/symbanded/BandedCholesky.jl:1-11,66-79), on the parent's band rows in place
+materialize!(M::MulAdd{<:ArrayLayouts.SymmetricLayout{HIPBC},HIPColumnMajor,HIPColumnMajor,<:Any,<:Any,<:ROCVector,<:ROCVector}) =
+    _sbmv!(ArrayLayouts.symmetricuplo(M.A), M.α, parent(M.A), M.B, M.β, M.C)
+function BandedMatrices.banded_chol!(::HIPBC, A::HIPBanded, ::Type{UpperTriangular})
+    info = _pbtrf!('U', size(A, 1), bandwidth(A, 2), bandeddata(A), 0)
+    UpperTriangular(A), info
+end
+function BandedMatrices.banded_chol!(::HIPBC, A::HIPBanded, ::Type{LowerTriangular})
+    info = _pbtrf!('L', size(A, 1), bandwidth(A, 1), bandeddata(A), bandwidth(A, 2))
+    LowerTriangular(A), info
+end
+function LinearAlgebra.ldiv!(C::Cholesky{T,<:HIPBanded{T}}, B::ROCVecOrMat{T}) where {T<:Union{Float32,Float64}}
+    F = C.factors; l, u = bandwidths(F)
+    C.uplo == 'U' ? _pbtrs!('U', size(F, 1), u, bandeddata(F), 0, B) : _pbtrs!('L', size(F, 1), l, bandeddata(F), u, B)
+end
 # Triangular views: the layouts triangularlayout(Tri, ::BandedColumns) gives
 # (src/tribanded.jl:9-11), with UPLO in ('U','L') and UNIT in ('N','U').
 materialize!(M::Ldiv{<:TriangularLayout{UPLO,UNIT,HIPBC},HIPColumnMajor,<:Any,<:ROCVector}) where {UPLO,UNIT} =
