@@ -56,6 +56,8 @@ for _p, _t in (("d", dbl), ("s", flt)):
     SIGNATURES[f"bh_{_p}gbcopy"] = (C.c_int, [i64, i64, vp, i64, i64, i64, vp, i64, i64, i64])
     SIGNATURES[f"bh_{_p}pbtrf"] = (C.c_int, [ch, i64, i64, vp, i64, pi64])
     SIGNATURES[f"bh_{_p}pbtrs"] = (C.c_int, [ch, i64, i64, i64, vp, i64, vp, i64])
+    SIGNATURES[f"bh_{_p}sbldlt"] = (C.c_int, [ch, i64, i64, vp, i64])
+    SIGNATURES[f"bh_{_p}sbldlts"] = (C.c_int, [ch, i64, i64, i64, vp, i64, vp, i64])
     SIGNATURES[f"bh_{_p}sbmv"] = (C.c_int, [ch, i64, i64, _t, vp, i64, vp, i64, _t, vp, i64])
 
 SIGNATURES["bh_batch_plan_create"] = (C.c_int, [i64, vp, vp, vp, C.POINTER(vp)])

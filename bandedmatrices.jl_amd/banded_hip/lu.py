@@ -123,13 +123,15 @@ def lu(A: BandedMatrix, check: bool = True) -> BandedLU:
 def ldiv_(F, B: DeviceArray) -> DeviceArray:
     """ldiv!(F, B) in place; F a BandedLU or transpose(F), or a triangular view
     of a BandedMatrix (UpperTriangular(A) etc., solved with tbsv)."""
-    from .symbanded import BandedCholesky, cholesky_ldiv_
+    from .symbanded import BandedCholesky, BandedLDLt, cholesky_ldiv_, ldlt_ldiv_
     from .tribanded import _Triangular
 
     if isinstance(F, _Triangular):
         return F.ldiv_(B)
     if isinstance(F, BandedCholesky):
         return cholesky_ldiv_(F, B)
+    if isinstance(F, BandedLDLt):
+        return ldlt_ldiv_(F, B)
     trans = "N"
     if isinstance(F, TransposeFact):
         F, trans = F.parent, "T"

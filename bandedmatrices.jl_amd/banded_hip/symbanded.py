@@ -5,6 +5,7 @@ mirroring the reference's symmetric banded layer.
   Symmetric(A) * x  (BLAS.sbmv!)                 src/symbanded/symbanded.jl:72-91
   cholesky(Symmetric(A)) (pbtrf!)                src/symbanded/BandedCholesky.jl:1-11, src/lapack.jl:268-292
   ldiv!(::Cholesky, B) (pbtrs!)                  src/symbanded/BandedCholesky.jl:66-79, src/lapack.jl:297-333
+  ldlt(Symmetric(A)), ldiv!(::LDLt, B)            src/symbanded/ldlt.jl:8-69
 """
 from __future__ import annotations
 
@@ -106,16 +107,55 @@ class BandedCholesky:
         return self.factors.u if self.uplo == "U" else self.factors.l
 
 
-def cholesky(S: Symmetric, check: bool = True) -> BandedCholesky:
-    """cholesky(Symmetric(A)): the triangle copied into its own (k+1) x n band
-    (bands (0, k) or (k, 0)) and factored in place by pbtrf."""
+@dataclass
+class BandedLDLt:
+    """factors: (k+1) x n storage of the triangle: L's entries where A's were, d on the diagonal."""
+
+    factors: BandedMatrix
+    uplo: str
+
+    @property
+    def shape(self):
+        return self.factors.shape
+
+    def kd(self) -> int:
+        return self.factors.u if self.uplo == "U" else self.factors.l
+
+
+def _triangle_copy(S: Symmetric) -> BandedMatrix:
     k = S.bandwidth()
     n = S.shape[0]
     bands = (0, k) if S.uplo == "U" else (k, 0)
     F = BandedMatrix.zeros((n, n), bands, S.dtype)  # corner slots outside the matrix hold 0
     l, u = S.parent.bandwidths()
     tri = BandedMatrix(S.symbandeddata().data.view(0 if S.uplo == "U" else u, 0, (k + 1, n)), n, *bands)
-    copyto_(F, tri)
+    return copyto_(F, tri)
+
+
+def ldlt(S: Symmetric) -> BandedLDLt:
+    """ldlt(Symmetric(A)): the triangle copied, then factored in place (no pivoting)."""
+    F = _triangle_copy(S)
+    _lib.call(f"bh_{_prefix(S.dtype)}sbldlt", S.uplo.encode(), S.shape[0], S.bandwidth(), F.data.ptr, F.data.ld)
+    return BandedLDLt(F, S.uplo)
+
+
+def ldlt_ldiv_(F: BandedLDLt, B: DeviceArray) -> DeviceArray:
+    """ldiv!(F, B) in place."""
+    n = F.shape[0]
+    if B.shape[0] != n:
+        raise DimensionMismatch(f"Matrix has dimensions {F.shape} but right hand side has first dimension {B.shape[0]}")
+    nrhs = 1 if len(B.shape) == 1 else B.shape[1]
+    _lib.call(f"bh_{_prefix(B.dtype)}sbldlts", F.uplo.encode(), n, F.kd(), nrhs, F.factors.data.ptr,
+              F.factors.data.ld, B.ptr, max(1, B.ld if len(B.shape) == 2 else n))
+    return B
+
+
+def cholesky(S: Symmetric, check: bool = True) -> BandedCholesky:
+    """cholesky(Symmetric(A)): the triangle copied into its own (k+1) x n band
+    (bands (0, k) or (k, 0)) and factored in place by pbtrf."""
+    k = S.bandwidth()
+    n = S.shape[0]
+    F = _triangle_copy(S)
     info = C.c_int64(0)
     _lib.call(f"bh_{_prefix(S.dtype)}pbtrf", S.uplo.encode(), n, k, F.data.ptr, F.data.ld, C.byref(info))
     if check and info.value > 0:

@@ -12,6 +12,11 @@
 // a_pq = fma(-v_q, v_p, a_pq).  Every entry sees the same operations in the
 // same order as on the host, so factor and info are bit-identical.
 //
+// The same kernel runs the banded LDL' of src/symbanded/ldlt.jl:8-37 (no
+// pivoting, no square root): d_j = a_jj, l = u / d_j by division, and the
+// trailing update a_pq = fma(-l_p, u_q, a_pq) -- the right-looking form of the
+// reference's left-looking loops, equal up to rounding.
+//
 // One workgroup walks the n steps with the kd+1 live columns in LDS (column c
 // in slot c mod (kd+1)); per step: scale the row (one thread per entry), a
 // barrier, the rank-1 update (one thread per column, which for column j+1
@@ -35,7 +40,7 @@ __device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_sqrtf(
 
 constexpr int kPbRegs = 24;  // staged values per thread: chunk columns x (kd+1) <= kPbRegs x threads
 
-template <typename T, bool UP>
+template <typename T, bool UP, bool LDL>
 __global__ void __launch_bounds__(256) pbtrf_lds_kernel(int n, int kd, T *__restrict__ ab, i64 ldab, i64 *__restrict__ info_out, int B) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int H = kd + 1;
@@ -44,9 +49,10 @@ __global__ void __launch_bounds__(256) pbtrf_lds_kernel(int n, int kd, T *__rest
     // fall in different LDS banks
     const int SL = H + (H & 1);
     T *W = reinterpret_cast<T *>(smem_raw);  // H slots of SL entries: the live columns
-    T *V = W + (i64)H * SL;                  // v_1 .. v_kd of the current step
-    T *S = V + H;                            // the next B incoming columns (column j+H at S[(j mod B)*H])
-    __shared__ T sR;                         // 1 / a_jj of the current step
+    T *V = W + (i64)H * SL;                  // v_1 .. v_kd of the current step (LDL: the unscaled u)
+    T *VL = LDL ? V + H : V;                 // LDL: l = u / d; Cholesky: the same v
+    T *S = V + (LDL ? 2 : 1) * H;            // the next B incoming columns (column j+H at S[(j mod B)*H])
+    __shared__ T sR;                         // Cholesky: 1 / a_jj; LDL: the pivot d_j
     __shared__ int sInfo;
     const int tid = threadIdx.x, nt = blockDim.x;
     const int dr = UP ? kd : 0;  // diagonal row
@@ -65,7 +71,9 @@ __global__ void __launch_bounds__(256) pbtrf_lds_kernel(int n, int kd, T *__rest
     __syncthreads();
     auto take_sqrt = [&](T *d, int info_if_bad) {
         const T v = *d;
-        if (v <= T(0)) {
+        if (LDL) {  // no root and no check: the pivot itself (a zero pivot gives Inf/NaN, as on the host)
+            sR = v;
+        } else if (v <= T(0)) {
             sInfo = info_if_bad;
         } else {
             const T s = dsqrt(v);
@@ -93,9 +101,16 @@ __global__ void __launch_bounds__(256) pbtrf_lds_kernel(int n, int kd, T *__rest
             // scale: v_c = U(j, j+c) (row kd-c of column j+c) or L(j+c, j) (row c of column j)
             for (int c = 1 + tid; c <= kn; c += nt) {
                 T *pv = UP ? slot(j + c) + (kd - c) : slot(j) + c;
-                const T v = *pv * sR;
-                *pv = v;
-                V[c] = v;
+                if (LDL) {
+                    const T u = *pv, l = u / sR;
+                    *pv = l;
+                    V[c] = u;
+                    VL[c] = l;
+                } else {
+                    const T v = *pv * sR;
+                    *pv = v;
+                    V[c] = v;
+                }
             }
             lds_barrier();
             // rank-1 update, thread per column q of the trailing triangle, in
@@ -112,13 +127,13 @@ __global__ void __launch_bounds__(256) pbtrf_lds_kernel(int n, int kd, T *__rest
                         T vv[8], aa[8];
 #pragma unroll
                         for (int u = 0; u < 8; ++u) {
-                            vv[u] = V[p + u];
+                            vv[u] = VL[p + u];
                             aa[u] = a0[p + u];
                         }
 #pragma unroll
                         for (int u = 0; u < 8; ++u) a0[p + u] = dfma(tq, vv[u], aa[u]);
                     }
-                    for (; p <= p1; ++p) a0[p] = dfma(tq, V[p], a0[p]);
+                    for (; p <= p1; ++p) a0[p] = dfma(tq, VL[p], a0[p]);
                 }
                 if (q == 1) take_sqrt(col + dr, j + 2);  // column j+1's diagonal is final
             }
@@ -148,7 +163,7 @@ __global__ void __launch_bounds__(256) pbtrf_lds_kernel(int n, int kd, T *__rest
     if (tid == 0) *info_out = sInfo;
 }
 
-template <typename T>
+template <typename T, bool LDL>
 int pbtrf_impl(char uplo, i64 n, i64 kd, T *ab, i64 ldab, i64 *info_host) {
     const bool up = (uplo == 'U' || uplo == 'u'), lo = (uplo == 'L' || uplo == 'l');
     BH_ARG(up || lo, 1);
@@ -162,7 +177,7 @@ int pbtrf_impl(char uplo, i64 n, i64 kd, T *ab, i64 ldab, i64 *info_host) {
     const int nt = (int)imin(256, 64 * cdiv(H, 64));
     // LDS: the live window, v, and a chunk of B incoming columns (B as large as
     // the registers and 150 KiB allow, at most 32)
-    const i64 fixed = (H * (H + (H & 1)) + H) * (i64)sizeof(T);
+    const i64 fixed = (H * (H + (H & 1)) + (LDL ? 2 : 1) * H) * (i64)sizeof(T);
     const i64 budget = 150 * 1024;
     const i64 B = imin(imin(32, (i64)kPbRegs * nt / H), (budget - fixed) / (H * (i64)sizeof(T)));
     if (B < 4 || n >= (1ll << 31)) {
@@ -174,7 +189,7 @@ int pbtrf_impl(char uplo, i64 n, i64 kd, T *ab, i64 ldab, i64 *info_host) {
     hipStream_t st = stream();
     i64 *dinfo = nullptr;
     BH_HIP(hipMallocAsync((void **)&dinfo, sizeof(i64), st));
-    auto kern = up ? pbtrf_lds_kernel<T, true> : pbtrf_lds_kernel<T, false>;
+    auto kern = up ? pbtrf_lds_kernel<T, true, LDL> : pbtrf_lds_kernel<T, false, LDL>;
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(kern, dim3(1), dim3(nt), lds, st, (int)n, (int)kd, ab, ldab, dinfo, (int)B);
@@ -205,6 +220,37 @@ int pbtrs_impl(char uplo, i64 n, i64 kd, i64 nrhs, const T *ab, i64 ldab, T *B, 
     int st = tbsv_run(up ? 2 : 1, ni, ki, ab, ldab, B, 1, ldb, nr, false);
     if (st != BH_OK) return st;
     return tbsv_run(up ? 0 : 3, ni, ki, ab, ldab, B, 1, ldb, nr, false);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) diag_div_kernel(int n, int nrhs, const T *__restrict__ ab, i64 ldab, int dr,
+                                                       T *__restrict__ B, i64 ldb) {
+    const i64 e = (i64)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (i64)n * nrhs) return;
+    const i64 c = e / n, j = e - c * n;
+    B[c * ldb + j] = B[c * ldb + j] / ab[j * ldab + dr];
+}
+
+// ldiv!(::LDLt, B): L y = B (unit), y ./= d, L' x = y (unit), columns in one
+// launch each; with 'U' storage L = U', so the sweeps are U' then U.
+template <typename T>
+int ldlts_impl(char uplo, i64 n, i64 kd, i64 nrhs, const T *ab, i64 ldab, T *B, i64 ldb) {
+    const bool up = (uplo == 'U' || uplo == 'u'), lo = (uplo == 'L' || uplo == 'l');
+    BH_ARG(up || lo, 1);
+    BH_ARG(n >= 0, 2);
+    BH_ARG(kd >= 0, 3);
+    BH_ARG(nrhs >= 0 && nrhs < (1ll << 31), 4);
+    BH_ARG(ldab >= kd + 1, 6);
+    BH_ARG(ldb >= (n > 1 ? n : 1), 8);
+    BH_ARG(n < (1ll << 30) && kd < (1ll << 30) && ldab * n * (i64)sizeof(T) < (1ll << 31) - (1ll << 24), 2);
+    if (n == 0 || nrhs == 0) return BH_OK;
+    const int ni = (int)n, ki = (int)kd, nr = (int)nrhs;
+    int st = tbsv_run(up ? 2 : 1, ni, ki, ab, ldab, B, 1, ldb, nr, true);
+    if (st != BH_OK) return st;
+    hipLaunchKernelGGL(diag_div_kernel<T>, dim3((unsigned)cdiv(n * nrhs, 256)), dim3(256), 0, stream(), ni, nr, ab,
+                       ldab, up ? ki : 0, B, ldb);
+    BH_LAUNCH_CHECK();
+    return tbsv_run(up ? 0 : 3, ni, ki, ab, ldab, B, 1, ldb, nr, true);
 }
 
 // ---- xSBMV: y <- alpha*A*x + beta*y, one thread per row; row i sums
@@ -258,10 +304,24 @@ using namespace bh;
 extern "C" {
 
 int bh_dpbtrf(char uplo, bh_int n, bh_int kd, double *AB, bh_int ldab, bh_int *info_host) {
-    return pbtrf_impl<double>(uplo, n, kd, AB, ldab, info_host);
+    return pbtrf_impl<double, false>(uplo, n, kd, AB, ldab, info_host);
 }
 int bh_spbtrf(char uplo, bh_int n, bh_int kd, float *AB, bh_int ldab, bh_int *info_host) {
-    return pbtrf_impl<float>(uplo, n, kd, AB, ldab, info_host);
+    return pbtrf_impl<float, false>(uplo, n, kd, AB, ldab, info_host);
+}
+int bh_dsbldlt(char uplo, bh_int n, bh_int kd, double *AB, bh_int ldab) {
+    bh_int info = 0;
+    return pbtrf_impl<double, true>(uplo, n, kd, AB, ldab, &info);
+}
+int bh_ssbldlt(char uplo, bh_int n, bh_int kd, float *AB, bh_int ldab) {
+    bh_int info = 0;
+    return pbtrf_impl<float, true>(uplo, n, kd, AB, ldab, &info);
+}
+int bh_dsbldlts(char uplo, bh_int n, bh_int kd, bh_int nrhs, const double *AB, bh_int ldab, double *B, bh_int ldb) {
+    return ldlts_impl<double>(uplo, n, kd, nrhs, AB, ldab, B, ldb);
+}
+int bh_ssbldlts(char uplo, bh_int n, bh_int kd, bh_int nrhs, const float *AB, bh_int ldab, float *B, bh_int ldb) {
+    return ldlts_impl<float>(uplo, n, kd, nrhs, AB, ldab, B, ldb);
 }
 int bh_dpbtrs(char uplo, bh_int n, bh_int kd, bh_int nrhs, const double *AB, bh_int ldab, double *B, bh_int ldb) {
     return pbtrs_impl<double>(uplo, n, kd, nrhs, AB, ldab, B, ldb);

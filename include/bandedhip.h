@@ -181,6 +181,19 @@ int bh_dpbtrs(char uplo, bh_int n, bh_int kd, bh_int nrhs, const double *AB, bh_
               bh_int ldb);
 int bh_spbtrs(char uplo, bh_int n, bh_int kd, bh_int nrhs, const float *AB, bh_int ldab, float *B,
               bh_int ldb);
+/* Banded LDL' without pivoting of the stored triangle (storage as for
+ * bh_?pbtrf), in place: L's entries where A's were, d on the diagonal.  No
+ * positivity check (a zero pivot gives Inf/NaN, as in the reference).  The
+ * right-looking form of the reference's loops: equal up to rounding.
+ * Replaces ldlt!(::Symmetric{<:Any,<:BandedMatrix}) (src/symbanded/ldlt.jl:8-44). */
+int bh_dsbldlt(char uplo, bh_int n, bh_int kd, double *AB, bh_int ldab);
+int bh_ssbldlt(char uplo, bh_int n, bh_int kd, float *AB, bh_int ldab);
+/* Solve with that factor: unit-L sweep, division by d, unit-L' sweep, all
+ * columns of B in one launch each.  Replaces ldiv!(::LDLt, B) (ldlt.jl:47-69). */
+int bh_dsbldlts(char uplo, bh_int n, bh_int kd, bh_int nrhs, const double *AB, bh_int ldab, double *B,
+                bh_int ldb);
+int bh_ssbldlts(char uplo, bh_int n, bh_int kd, bh_int nrhs, const float *AB, bh_int ldab, float *B,
+                bh_int ldb);
 /* y <- alpha*A*x + beta*y for the symmetric band A stored as one triangle (as
  * above, k off-diagonals); y not read when beta == 0.  Row i sums its terms
  * in column order with one fma each.  Replaces BLAS.sbmv! as

@@ -15,6 +15,7 @@
  *   - BLAS ?tbsv upper              (src/tribanded.jl:75-83, src/blas.jl:123-141)
  *   - BLAS ?tbsv / ?tbmv, all cases (src/tribanded.jl:47-54,75-96, src/blas.jl:71-147)
  *   - LAPACK ?pbtf2 and BLAS ?sbmv  (src/symbanded/BandedCholesky.jl:1-11, symbanded.jl:72-91)
+ *   - banded LDL' and its solve     (src/symbanded/ldlt.jl:8-69)
  *   - banded Householder QR         (src/banded/bandedqr.jl:8-35) with the
  *     reflector!/reflectorApply! pair it imports from ArrayLayouts
  *   - Q / Q' application and solve  (src/banded/bandedqr.jl:38-94,175-206)
@@ -394,6 +395,51 @@ void bo_stbmv(char uplo, char trans, char diag, i64 n, i64 k, const float *a, i6
 
 i64 bo_dpbtf2(char uplo, i64 n, i64 kd, double *ab, i64 ldab) { PBTF2_BODY(double, sqrt, fma) }
 i64 bo_spbtf2(char uplo, i64 n, i64 kd, float *ab, i64 ldab) { PBTF2_BODY(float, sqrtf, fmaf) }
+
+/* Banded LDL' without pivoting, restating ldlt! (src/symbanded/ldlt.jl:8-37)
+ * on the stored triangle (storage as for pbtf2), left-looking in j with each
+ * sum taken in ascending k (the reference's @simd loops leave the order to
+ * the compiler, so results are defined to rounding):
+ *   d_j = a_jj - sum_k l_jk^2 d_k,  l_ij = (a_ij - sum_k l_ik l_jk d_k) / d_j.
+ * The solve restates ldiv!(::LDLt, B) (ldlt.jl:47-69): forward with unit L,
+ * divide by d, backward with L'. */
+#define LDL_AT(r, c) (up ? ab[IDX(ldab, kd + (r) - (c), (c))] : ab[IDX(ldab, (c) - (r), (r))]) /* A(max, min) */
+#define SBLDLT_BODY(T)                                                                  \
+    const int up = (uplo == 'U' || uplo == 'u');                                        \
+    /* element (i, j), i >= j, of the lower factor: L(i,j) lives where A(i,j) does */     \
+    for (i64 j = 0; j < n; ++j) {                                                       \
+        T *djj = up ? &ab[IDX(ldab, kd, j)] : &ab[IDX(ldab, 0, j)];                     \
+        for (i64 k = imax(0, j - kd); k < j; ++k) {                                     \
+            const T ljk = up ? ab[IDX(ldab, kd + k - j, j)] : ab[IDX(ldab, j - k, k)];  \
+            const T dk = up ? ab[IDX(ldab, kd, k)] : ab[IDX(ldab, 0, k)];               \
+            *djj -= ljk * ljk * dk;                                                     \
+        }                                                                               \
+        for (i64 i = j + 1; i <= imin(n - 1, j + kd); ++i) {                            \
+            T *aij = up ? &ab[IDX(ldab, kd + j - i, i)] : &ab[IDX(ldab, i - j, j)];     \
+            for (i64 k = imax(0, i - kd); k < j; ++k) {                                 \
+                const T lik = up ? ab[IDX(ldab, kd + k - i, i)] : ab[IDX(ldab, i - k, k)]; \
+                const T ljk = up ? ab[IDX(ldab, kd + k - j, j)] : ab[IDX(ldab, j - k, k)]; \
+                const T dk = up ? ab[IDX(ldab, kd, k)] : ab[IDX(ldab, 0, k)];           \
+                *aij -= lik * ljk * dk;                                                 \
+            }                                                                           \
+            *aij /= *djj;                                                               \
+        }                                                                               \
+    }
+
+void bo_dsbldlt(char uplo, i64 n, i64 kd, double *ab, i64 ldab) { SBLDLT_BODY(double) }
+void bo_ssbldlt(char uplo, i64 n, i64 kd, float *ab, i64 ldab) { SBLDLT_BODY(float) }
+
+#define SBLDLTS_BODY(T)                                                                 \
+    const int up = (uplo == 'U' || uplo == 'u');                                        \
+    for (i64 j = 0; j < n; ++j)                                                         \
+        for (i64 k = imax(0, j - kd); k < j; ++k)                                       \
+            b[j] -= (up ? ab[IDX(ldab, kd + k - j, j)] : ab[IDX(ldab, j - k, k)]) * b[k]; \
+    for (i64 j = 0; j < n; ++j) b[j] /= (up ? ab[IDX(ldab, kd, j)] : ab[IDX(ldab, 0, j)]); \
+    for (i64 j = n - 1; j >= 0; --j)                                                    \
+        for (i64 k = j + 1; k <= imin(n - 1, j + kd); ++k)                              \
+            b[j] -= (up ? ab[IDX(ldab, kd + j - k, k)] : ab[IDX(ldab, k - j, j)]) * b[k];
+
+void bo_dsbldlts(char uplo, i64 n, i64 kd, const double *ab, i64 ldab, double *b) { SBLDLTS_BODY(double) }
 
 /* Symmetric band product y <- alpha*A*x + beta*y (BLAS xSBMV, as
  * src/symbanded/symbanded.jl:72-91 calls it), A stored as one triangle like

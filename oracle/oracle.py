@@ -63,9 +63,14 @@ def lib():
             f = getattr(L, f"bo_{pre}pbtf2")
             f.argtypes = [C.c_char, i64, i64, fp, i64]
             f.restype = i64
+            f = getattr(L, f"bo_{pre}sbldlt")
+            f.argtypes = [C.c_char, i64, i64, fp, i64]
+            f.restype = None
             f = getattr(L, f"bo_{pre}sbmv")
             f.argtypes = [C.c_char, i64, i64, ft, fp, i64, fp, ft, fp]
             f.restype = None
+        L.bo_dsbldlts.argtypes = [C.c_char, i64, i64, _dp, i64, _dp]
+        L.bo_dsbldlts.restype = None
         L.bo_dgbtf2_planted.argtypes = [C.c_int, i64, i64, i64, i64, _dp, i64, _ip]
         L.bo_dgbtf2_planted.restype = i64
         L.bo_dgbmm.argtypes = [i64, i64, i64, C.c_double, _dp, i64, i64, i64, _dp, i64, i64, i64,
@@ -196,6 +201,22 @@ def pbtrs(uplo, n, kd, f, b):
         return tbsv("U", "N", "N", n, kd, f, y)
     y = tbsv("L", "N", "N", n, kd, f, b)
     return tbsv("L", "T", "N", n, kd, f, y)
+
+
+def sbldlt(uplo, n, kd, ab):
+    """Banded LDL' of the stored triangle (ldlt!): L's entries where A's were, d on the diagonal."""
+    pre, P = _pre(ab.dtype)
+    f = _f(ab).copy(order="F")
+    getattr(lib(), f"bo_{pre}sbldlt")(uplo.encode(), n, kd, _ptr(f, P), f.shape[0])
+    return f
+
+
+def sbldlts(uplo, n, kd, f, b):
+    """ldiv!(::LDLt, b) with the factor of sbldlt (FP64)."""
+    ff = _f(f)
+    bb = np.ascontiguousarray(b, dtype=np.float64).copy()
+    lib().bo_dsbldlts(uplo.encode(), n, kd, _ptr(ff, _dp), ff.shape[0], _ptr(bb, _dp))
+    return bb
 
 
 def sbmv(uplo, n, k, alpha, ab, x, beta=0.0, y=None):

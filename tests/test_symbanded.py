@@ -114,6 +114,38 @@ def test_oracle_sbmv_vs_blas(oracle, uplo):
         assert np.abs(oracle.sbmv(uplo, n, k, 1.5, a, x) - ref).max() <= 1e-14 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("uplo", ["U", "L"])
+def test_oracle_ldlt_closed_form_and_identity(oracle, uplo):
+    """[-1, 2, -1]: d_j = (j+2)/(j+1), l = -(j+1)/(j+2) (0-based); L D L' = A and
+    the solve on a random diagonally dominant band."""
+    n = 12
+    a = np.zeros((2, n), order="F")
+    if uplo == "U":
+        a[1, :], a[0, 1:] = 2, -1
+    else:
+        a[0, :], a[1, :-1] = 2, -1
+    f = oracle.sbldlt(uplo, n, 1, a)
+    j = np.arange(n)
+    d, lv = (f[1], f[0, 1:]) if uplo == "U" else (f[0], f[1, :-1])
+    assert np.allclose(d, (j + 2) / (j + 1), rtol=1e-15)
+    assert np.allclose(lv, -(j[:-1] + 1) / (j[:-1] + 2), rtol=1e-15)
+    rng = np.random.default_rng(5)
+    n, kd = 80, 6
+    a = spd_band(rng, n, kd, uplo)
+    f = oracle.sbldlt(uplo, n, kd, a)
+    L = np.eye(n)
+    D = np.zeros(n)
+    for c in range(n):
+        D[c] = f[kd, c] if uplo == "U" else f[0, c]
+        for t in range(1, kd + 1):
+            if c + t < n:
+                L[c + t, c] = f[kd - t, c + t] if uplo == "U" else f[t, c]
+    M = sym_dense(a, n, kd, uplo)
+    assert np.allclose(L @ np.diag(D) @ L.T, M, rtol=1e-13, atol=1e-13)
+    b = rng.standard_normal(n)
+    assert np.allclose(oracle.sbldlts(uplo, n, kd, f, b), np.linalg.solve(M, b), rtol=1e-12, atol=1e-13)
+
+
 def test_argument_errors():
     from banded_hip import _lib
     import ctypes
@@ -125,6 +157,8 @@ def test_argument_errors():
     assert L.bh_dpbtrf(b"U", 4, 2, None, 2, ctypes.byref(info)) == -5
     assert L.bh_dpbtrs(b"U", 4, 1, 1, None, 2, None, 3) == -8
     assert L.bh_dsbmv(b"U", 4, 1, 1.0, None, 2, None, 0, 0.0, None, 1) == -8
+    assert L.bh_dsbldlt(b"U", 4, 3, None, 3) == -5
+    assert L.bh_dsbldlts(b"L", 4, 1, 1, None, 2, None, 2) == -8
 
 
 # ------------------------------------------------------------------ GPU
@@ -223,6 +257,38 @@ def test_gpu_sbmv_bitexact(bh, oracle, uplo, dtype):
         A, X, Y = bh.DeviceArray.from_numpy(a), bh.DeviceArray.from_numpy(x), bh.DeviceArray.from_numpy(y0)
         bh._lib.call(f"bh_{pre}sbmv", uplo.encode(), n, k, dtype(0.75), A.ptr, A.ld, X.ptr, 1, dtype(-2.0), Y.ptr, 1)
         assert bits_equal(Y.numpy(), oracle.sbmv(uplo, n, k, dtype(0.75), a, x, dtype(-2.0), y0)), (n, k)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("uplo", ["U", "L"])
+def test_gpu_ldlt(bh, oracle, uplo):
+    """ldlt(Symmetric(A)) \\ B: the factor against the oracle's restatement of the
+    reference's loops to rounding (the update order differs), the solve against
+    the oracle and a dense solve."""
+    rng = np.random.default_rng(16)
+    for n, l, u in [(500, 4, 4), (300, 40, 40), (64, 0, 0)]:
+        D = np.asfortranarray(rng.uniform(-1, 1, size=(l + u + 1, n)))
+        D[u, :] = 2 * (max(l, u) + 1) + rng.random(n)
+        A = bh.BandedMatrix.from_host(D, n, l, u)
+        S = bh.Symmetric(A, uplo)
+        F = bh.ldlt(S)
+        kd = S.bandwidth()
+        tri = np.asfortranarray(D[: u + 1] if uplo == "U" else D[u:]).copy()
+        for j in range(n):  # the corner slots the mirror zeroes
+            for t in range(1, kd + 1):
+                if uplo == "U" and j - t < 0:
+                    tri[kd - t, j] = 0
+                if uplo == "L" and j + t >= n:
+                    tri[t, j] = 0
+        f = oracle.sbldlt(uplo, n, kd, tri)
+        got = F.factors.host_data()
+        assert np.abs(got - f).max() <= 1e-13 * np.abs(f).max(), (n, l, u)
+        B = rng.standard_normal((n, 2))
+        X = bh.ldiv_(F, bh.DeviceArray.from_numpy(B)).numpy()
+        for c in range(2):
+            xo = oracle.sbldlts(uplo, n, kd, f, B[:, c])
+            assert np.abs(X[:, c] - xo).max() <= 1e-12 * np.abs(xo).max()
+        assert np.allclose(S.to_dense() @ X, B, rtol=1e-11, atol=1e-11)
 
 
 @pytest.mark.gpu
