@@ -18,6 +18,7 @@
 #   copyto!(B, A) banded   copyto_bandeddata!                          src/banded/BandedMatrix.jl:124-130
 #   Symmetric(A) * x       BLAS.sbmv!                                  src/symbanded/symbanded.jl:72-91
 #   cholesky / ldiv!       pbtrf! / pbtrs!                             src/symbanded/BandedCholesky.jl:1-79
+#   ldlt! / ldiv!          banded LDLᵀ and its solve                   src/symbanded/ldlt.jl:8-69
 module BandedMatricesHIPExt
 
 using BandedMatrices, LinearAlgebra, ArrayLayouts, AMDGPU
@@ -69,7 +70,31 @@ for (T, p) in ((Float64, :d), (Float32, :s))
     tbsv = Symbol(:bh_, p, :tbsv); tbmv = Symbol(:bh_, p, :tbmv)
     gbscal = Symbol(:bh_, p, :gbscal); gbaxpy = Symbol(:bh_, p, :gbaxpy); gbcopy = Symbol(:bh_, p, :gbcopy)
     pbtrf = Symbol(:bh_, p, :pbtrf); pbtrs = Symbol(:bh_, p, :pbtrs); sbmv = Symbol(:bh_, p, :sbmv)
+    sbldlt = Symbol(:bh_, p, :sbldlt); sbldlts = Symbol(:bh_, p, :sbldlts)
     @eval begin
+        # ldlt! in place on the stored triangle (src/symbanded/ldlt.jl:8-37)
+        function _sbldlt!(uplo::Char, A::HIPBanded{$T})
+            l, u = bandwidths(A); d = bandeddata(A)
+            k, off = uplo == 'U' ? (u, 0) : (l, u)
+            with_stream() do
+                chk(ccall(($(QuoteNode(sbldlt)), libbh), Cint, (UInt8, Int64, Int64, Ptr{$T}, Int64),
+                          uplo, size(A, 1), k, dptr(d) + off * sizeof($T), stride(d, 2)), $(QuoteNode(sbldlt)))
+            end
+            A
+        end
+        # ldiv!(::LDLt, B) (src/symbanded/ldlt.jl:47-69)
+        function _sbldlts!(uplo::Char, A::HIPBanded{$T}, B::ROCVecOrMat{$T})
+            l, u = bandwidths(A); d = bandeddata(A)
+            k, off = uplo == 'U' ? (u, 0) : (l, u)
+            ldb = B isa ROCVector ? length(B) : stride(B, 2)
+            with_stream() do
+                chk(ccall(($(QuoteNode(sbldlts)), libbh), Cint,
+                          (UInt8, Int64, Int64, Int64, Ptr{$T}, Int64, Ptr{$T}, Int64),
+                          uplo, size(A, 1), k, size(B, 2), dptr(d) + off * sizeof($T), stride(d, 2), dptr(B), ldb),
+                    $(QuoteNode(sbldlts)))
+            end
+            B
+        end
         # pbtrf!(uplo, n, kd, AB) on the triangle's band rows (src/lapack.jl:268-292)
         function _pbtrf!(uplo::Char, n::Int, kd::Int, AB::ROCMatrix{$T}, off::Int)
             info = Ref{Int64}(0)
@@ -254,6 +279,12 @@ function BandedMatrices.banded_chol!(::HIPBC, A::HIPBanded, ::Type{LowerTriangul
     info = _pbtrf!('L', size(A, 1), bandwidth(A, 1), bandeddata(A), bandwidth(A, 2))
     LowerTriangular(A), info
 end
+function LinearAlgebra.ldlt!(F::Symmetric{T,<:HIPBanded{T}}) where {T<:Union{Float32,Float64}}
+    _sbldlt!(F.uplo, F.data)
+    LDLt{T,typeof(F)}(F)
+end
+LinearAlgebra.ldiv!(S::LDLt{T,<:Symmetric{T,<:HIPBanded{T}}}, B::ROCVecOrMat{T}) where {T<:Union{Float32,Float64}} =
+    _sbldlts!(S.data.uplo, S.data.data, B)
 function LinearAlgebra.ldiv!(C::Cholesky{T,<:HIPBanded{T}}, B::ROCVecOrMat{T}) where {T<:Union{Float32,Float64}}
     F = C.factors; l, u = bandwidths(F)
     C.uplo == 'U' ? _pbtrs!('U', size(F, 1), u, bandeddata(F), 0, B) : _pbtrs!('L', size(F, 1), l, bandeddata(F), u, B)
