@@ -11,6 +11,8 @@
 // coalesced stores.  Bands of C outside the product's bands are beta-scaled
 // (gbmm.jl:286-288) and columns the product never reaches are beta-scaled
 // (gbmm.jl:339), exactly as the reference does.
+#include <stdlib.h>
+
 #include <vector>
 
 #include "bh_common.hpp"
@@ -213,8 +215,17 @@ static int gbmm_impl(i64 n, i64 k, i64 m, T alpha, const T *A, i64 lda, i64 Al, 
     }
     GbmmArgs<T> p{n, k, m, alpha, beta, A, lda, Al, Au, B, ldb, Bl, Bu, C, ldc, Ctl, Ctu, Cl, Cu};
     const i64 hA = Al + Au + 1, hB = Bl + Bu + 1;
-    // columns per workgroup: keep the LDS slabs within 64 KiB
-    i64 NC = 256;
+    // columns per workgroup: 64 (measured at n = 2^20, (8,8)x(4,4): 3.1 TB/s
+    // against 1.5 TB/s at 256, where 54 KiB of LDS per workgroup left two
+    // workgroups per CU), halved while the launch would have fewer than 512
+    // workgroups, and kept within 64 KiB of LDS.  The column count changes no
+    // result.  BANDEDHIP_GBMM_NC overrides the start (a power of two <= 256).
+    i64 NC = 64;
+    while (NC > 8 && cdiv(m, NC) < 512) NC /= 2;
+    if (const char *e = getenv("BANDEDHIP_GBMM_NC")) {
+        const i64 v = atoll(e);
+        if (v >= 1 && v <= 256 && (v & (v - 1)) == 0) NC = v;
+    }
     auto lds_for = [&](i64 nc) { return (size_t)(nc * odd_stride(hB) + (nc + Bl + Bu) * odd_stride(hA)) * sizeof(T); };
     while (NC > 1 && lds_for(NC) > 64 * 1024) NC /= 2;
     const size_t lds = lds_for(NC);
