@@ -8,6 +8,9 @@ generated here:
   dgeqrf the reference's BLAS.gbmv! / LAPACK.gbtrf! / LAPACK.gbtrs! calls
   reach -- on small seeded inputs, with NaN in the corner slots of the band
   storage (never read), ties in pivot columns and a singular column;
+* for the triangular and symmetric rows, from scipy's dtbsv / dtbmv / dpbtrf /
+  dpbtrs / dsbmv (the routines tbsv!, tbmv!, pbtrf!, pbtrs! and BLAS.sbmv!
+  call, src/blas.jl:71-147, src/lapack.jl:268-333, src/symbanded/symbanded.jl:72-91);
 * from the reference tests' deterministic integer cases, whose products are
   exact: the symmetric band of test/test_linalg.jl:54 and the
   (-1, 1, 2)-banded matrix of test/test_linalg.jl:99, and the right-hand side
@@ -154,11 +157,63 @@ def qr_fixtures():
     np.savez(os.path.join(HERE, "qr_lapack.npz"), **out)
 
 
+def triangular_fixtures():
+    rng = np.random.default_rng(21)
+    out = {}
+    for uplo in ("U", "L"):
+        for diag in ("N", "U"):
+            n, k = 90, 5
+            a = rng.uniform(-1.0, 1.0, size=(k + 1, n)) * 0.1
+            dr = k if uplo == "U" else 0
+            a[dr, :] = 1.0 + rng.random(n)
+            a = np.asfortranarray(a)
+            x = rng.standard_normal(n)
+            tag = f"{uplo}{diag}"
+            out[f"{tag}_dims"] = np.array([n, k])
+            out[f"{tag}_a"] = a
+            out[f"{tag}_x"] = x
+            for trans in ("N", "T"):
+                kw = dict(lower=int(uplo == "L"), trans=int(trans == "T"), diag=int(diag == "U"))
+                out[f"{tag}_sv{trans}"] = sblas.dtbsv(k, a, x.copy(), **kw)
+                out[f"{tag}_mv{trans}"] = sblas.dtbmv(k, a, x.copy(), **kw)
+    np.savez(os.path.join(HERE, "tri_openblas.npz"), **out)
+
+
+def symmetric_fixtures():
+    rng = np.random.default_rng(22)
+    out = {}
+    for uplo in ("U", "L"):
+        for n, kd in ((70, 4), (120, 20)):
+            a = rng.uniform(-1.0, 1.0, size=(kd + 1, n))
+            dr = kd if uplo == "U" else 0
+            a[dr, :] = kd + 1 + rng.random(n)
+            for j in range(n):  # corner slots outside the matrix hold 0
+                for t in range(1, kd + 1):
+                    if uplo == "U" and j - t < 0:
+                        a[kd - t, j] = 0
+                    if uplo == "L" and j + t >= n:
+                        a[t, j] = 0
+            a = np.asfortranarray(a)
+            c, info = slap.dpbtrf(a.copy(order="F"), lower=int(uplo == "L"))
+            b = rng.standard_normal(n)
+            xs, _ = slap.dpbtrs(c, b, lower=int(uplo == "L"))
+            x = rng.standard_normal(n)
+            y = sblas.dsbmv(kd, 1.25, a, x, lower=int(uplo == "L"))
+            tag = f"{uplo}{kd}"
+            out.update({f"{tag}_dims": np.array([n, kd]), f"{tag}_a": a, f"{tag}_chol": np.asfortranarray(c),
+                        f"{tag}_info": np.array([info]), f"{tag}_b": b, f"{tag}_x": xs, f"{tag}_sx": x,
+                        f"{tag}_sy": y})
+    np.savez(os.path.join(HERE, "sym_lapack.npz"), **out)
+
+
 if __name__ == "__main__":
-    gbmv_fixtures()
-    lu_fixtures()
-    integer_fixtures()
-    qr_fixtures()
+    import sys
+
+    only = sys.argv[1:]  # e.g. "triangular symmetric": regenerate only those
+    for name, fn in (("gbmv", gbmv_fixtures), ("lu", lu_fixtures), ("integer", integer_fixtures),
+                     ("qr", qr_fixtures), ("triangular", triangular_fixtures), ("symmetric", symmetric_fixtures)):
+        if not only or name in only:
+            fn()
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)), "bytes")

@@ -5,7 +5,9 @@ tests' exact integer cases) against the oracle (CPU) and the library (GPU).
 Bars: gbmv 'N', gbtrf factors + pivots, gbtrs 'N' bit for bit (the reference
 calls the same OpenBLAS routines the fixtures come from); gbtrs 'T' 1e-13;
 integer closed forms exactly; QR's R and tau against dgeqrf to 1e-12 (the
-Householder convention is the same, the operation order is not)."""
+Householder convention is the same, the operation order is not); tbsv 'N' and
+pbtrf bit for bit, tbsv 'T', tbmv, pbtrs and sbmv to 1e-13 (OpenBLAS blocks
+those sums)."""
 import os
 
 import numpy as np
@@ -139,3 +141,76 @@ def test_gpu_integer_closed_forms(bh):
     ref = z["sym_MM"]
     inside = ~np.isnan(ref)  # corner slots of band storage lie outside the matrix and are never written
     assert np.array_equal(C.data.numpy()[inside], ref[inside])
+
+
+TRI_TAGS = ["UN", "UU", "LN", "LU"]
+SYM_TAGS = ["U4", "U20", "L4", "L20"]
+
+
+def _close(got, ref, tol=1e-13):
+    return np.abs(got - ref).max() <= tol * max(1.0, np.abs(ref).max())
+
+
+@pytest.mark.parametrize("tag", TRI_TAGS)
+def test_oracle_triangular_golden(oracle, tag):
+    z = load("tri_openblas.npz")
+    n, k = (int(v) for v in z[f"{tag}_dims"])
+    uplo, diag = tag[0], tag[1]
+    a, x = z[f"{tag}_a"], z[f"{tag}_x"]
+    assert bits_equal(oracle.tbsv(uplo, "N", diag, n, k, a, x), z[f"{tag}_svN"])
+    assert _close(oracle.tbsv(uplo, "T", diag, n, k, a, x), z[f"{tag}_svT"])
+    for t in "NT":
+        assert _close(oracle.tbmv(uplo, t, diag, n, k, a, x), z[f"{tag}_mv{t}"])
+
+
+@pytest.mark.parametrize("tag", SYM_TAGS)
+def test_oracle_symmetric_golden(oracle, tag):
+    z = load("sym_lapack.npz")
+    n, kd = (int(v) for v in z[f"{tag}_dims"])
+    uplo = tag[0]
+    f, info = oracle.pbtf2(uplo, n, kd, z[f"{tag}_a"])
+    assert info == int(z[f"{tag}_info"][0]) == 0
+    assert bits_equal(f, z[f"{tag}_chol"])
+    assert _close(oracle.pbtrs(uplo, n, kd, f, z[f"{tag}_b"]), z[f"{tag}_x"])
+    assert _close(oracle.sbmv(uplo, n, kd, 1.25, z[f"{tag}_a"], z[f"{tag}_sx"]), z[f"{tag}_sy"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tag", TRI_TAGS)
+def test_gpu_triangular_golden(bh, tag):
+    z = load("tri_openblas.npz")
+    n, k = (int(v) for v in z[f"{tag}_dims"])
+    uplo, diag = tag[0], tag[1]
+    A = bh.DeviceArray.from_numpy(z[f"{tag}_a"])
+    for t in "NT":
+        X = bh.DeviceArray.from_numpy(z[f"{tag}_x"])
+        bh.tbsv_(uplo, t, diag, n, k, A, X)
+        if t == "N":
+            assert bits_equal(X.numpy(), z[f"{tag}_svN"])
+        else:
+            assert _close(X.numpy(), z[f"{tag}_svT"])
+        X = bh.DeviceArray.from_numpy(z[f"{tag}_x"])
+        bh.tbmv_(uplo, t, diag, n, k, A, X)
+        assert _close(X.numpy(), z[f"{tag}_mv{t}"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tag", SYM_TAGS)
+def test_gpu_symmetric_golden(bh, tag):
+    import ctypes
+
+    z = load("sym_lapack.npz")
+    n, kd = (int(v) for v in z[f"{tag}_dims"])
+    uplo = tag[0].encode()
+    A = bh.DeviceArray.from_numpy(z[f"{tag}_a"])
+    info = ctypes.c_int64(-1)
+    bh._lib.call("bh_dpbtrf", uplo, n, kd, A.ptr, A.ld, ctypes.byref(info))
+    assert info.value == 0 and bits_equal(A.numpy(), z[f"{tag}_chol"])
+    B = bh.DeviceArray.from_numpy(z[f"{tag}_b"])
+    bh._lib.call("bh_dpbtrs", uplo, n, kd, 1, A.ptr, A.ld, B.ptr, n)
+    assert _close(B.numpy(), z[f"{tag}_x"])
+    A0 = bh.DeviceArray.from_numpy(z[f"{tag}_a"])
+    X = bh.DeviceArray.from_numpy(z[f"{tag}_sx"])
+    Y = bh.DeviceArray((n,), np.float64)
+    bh._lib.call("bh_dsbmv", uplo, n, kd, 1.25, A0.ptr, A0.ld, X.ptr, 1, 0.0, Y.ptr, 1)
+    assert _close(Y.numpy(), z[f"{tag}_sy"])
