@@ -75,23 +75,40 @@ __global__ void __launch_bounds__(NT) gbmv_n_slab(int m, int n, int kl, int ku, 
     const i64 abytes = a_elems * (i64)sizeof(T);
     const unsigned char *gsrc = reinterpret_cast<const unsigned char *>(A) + ga;
     uint4 pre[NPF];
+    // Every chunk inside A (all workgroups but those at A's two ends): one
+    // unconditional 16-byte buffer load per chunk, chunks past the slab read
+    // as zero, so the compiler has no branch to merge and waits only once.
+    typedef unsigned sv4u __attribute__((ext_vector_type(4)));
+    const bool inside = ga >= 0 && ga + 16 * (i64)nchunks <= abytes && abytes < (1ll << 31) - (1ll << 24);
+    if (inside) {
+        const __amdgpu_buffer_rsrc_t ra =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<T *>(A), 0, (int)abytes, 0x00020000);
 #pragma unroll
-    for (int q = 0; q < NPF; ++q) {
-        const int k = tid + q * NT;
-        const i64 off = ga + 16 * (i64)k;
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (k < nchunks) {
-            if (off >= 0 && off + 16 <= abytes) {
-                v = *reinterpret_cast<const uint4 *>(gsrc + 16 * (i64)k);
-            } else {
-                unsigned w[4] = {0, 0, 0, 0};
-                for (int b = 0; b < 4; ++b)
-                    if (off + 4 * b >= 0 && off + 4 * b + 4 <= abytes)
-                        w[b] = *reinterpret_cast<const unsigned *>(gsrc + 16 * (i64)k + 4 * b);
-                v = make_uint4(w[0], w[1], w[2], w[3]);
-            }
+        for (int q = 0; q < NPF; ++q) {
+            const int k = tid + q * NT;
+            const int o = k < nchunks ? (int)(ga + 16 * (i64)k) : 0x7f800000;  // past the end: reads 0
+            const sv4u v = __builtin_amdgcn_raw_buffer_load_b128(ra, o, 0, 0);
+            pre[q] = make_uint4(v.x, v.y, v.z, v.w);
         }
-        pre[q] = v;
+    } else {
+#pragma unroll
+        for (int q = 0; q < NPF; ++q) {
+            const int k = tid + q * NT;
+            const i64 off = ga + 16 * (i64)k;
+            uint4 v = make_uint4(0, 0, 0, 0);
+            if (k < nchunks) {
+                if (off >= 0 && off + 16 <= abytes) {
+                    v = *reinterpret_cast<const uint4 *>(gsrc + 16 * (i64)k);
+                } else {
+                    unsigned w[4] = {0, 0, 0, 0};
+                    for (int b = 0; b < 4; ++b)
+                        if (off + 4 * b >= 0 && off + 4 * b + 4 <= abytes)
+                            w[b] = *reinterpret_cast<const unsigned *>(gsrc + 16 * (i64)k + 4 * b);
+                    v = make_uint4(w[0], w[1], w[2], w[3]);
+                }
+            }
+            pre[q] = v;
+        }
     }
     constexpr int XPT = RPT + 1;  // x values per thread (nc <= R + kl + ku <= XPT * NT when kl+ku <= NT)
     constexpr int XD = 8;         // right-hand sides whose x is in flight
@@ -99,12 +116,23 @@ __global__ void __launch_bounds__(NT) gbmv_n_slab(int m, int n, int kl, int ku, 
     // right-hand sides ahead of their use, so with many right-hand sides the
     // x loads overlap the products instead of waiting once per right-hand side
     T xq[XD][XPT];
+    // Each right-hand side's x comes through a buffer resource covering just
+    // its nc slab entries (none past the last right-hand side): every load is
+    // unconditional, lanes past the range read 0, and the compiler can count
+    // the loads in flight instead of waiting for all of them.
     auto load_x = [&](T(&xv)[XPT], int r) __attribute__((always_inline)) {
-        const T *x = X + (i64)r * ldx + c0;
+        const bool live = r < nrhs;
+        int xb = live ? nc * (int)sizeof(T) : 0;
+        asm volatile("" : "+s"(xb));
+        const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<T *>(X) + (live ? (i64)r * ldx + c0 : 0), 0, xb, 0x00020000);
         CFor<XPT>::run([&](auto qi) {
             constexpr int q = decltype(qi)::value;
             const int c = tid + q * NT;
-            xv[q] = (r < nrhs && c < nc) ? x[c] : T(0);
+            if constexpr (sizeof(T) == 8)
+                xv[q] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rx, c * 8, 0, 0));
+            else
+                xv[q] = __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rx, c * 4, 0, 0));
         });
     };
     CFor<XD>::run([&](auto di) {
