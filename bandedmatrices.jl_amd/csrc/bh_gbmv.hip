@@ -32,23 +32,53 @@ static size_t tile_lds_bytes(i64 R, i64 kl, i64 ku, int nvec) {
     return (size_t)(W * slab_stride(kl, ku) + (i64)nvec * W) * sizeof(T);
 }
 
-// Stage band columns [c0,c1) (all kl+ku+1 stored rows) into the slab.
+// Stage band columns [c0,c1) (all kl+ku+1 stored rows) into the slab.  Each
+// thread issues kStageBatch loads before it stores any (load indices clamped
+// to the last element, so no load is conditional): a workgroup waits for
+// memory a few times, not once per element.
+constexpr int kStageBatch = 8;
+
 template <typename T>
 __device__ __forceinline__ void load_slab(T *__restrict__ slab, i64 S, const T *__restrict__ A, i64 lda,
                                           i64 c0, i64 c1, i64 h) {
     // The slab holds at most 64 KiB, so 32-bit indexing is exact here.
     const int total = (int)((c1 - c0) * h);
+    if (total <= 0) return;
     const unsigned hh = (unsigned)h;
+    const int nt = blockDim.x;
+    const unsigned Su = (unsigned)S;
     if (lda == h) {
         const T *src = A + c0 * lda;
-        for (int e = threadIdx.x; e < total; e += blockDim.x) {
-            unsigned c = (unsigned)e / hh, t = (unsigned)e - c * hh;
-            slab[c * S + t] = src[e];
+        for (int base = threadIdx.x; base < total; base += kStageBatch * nt) {
+            T v[kStageBatch];
+#pragma unroll
+            for (int u = 0; u < kStageBatch; ++u) v[u] = src[min(base + u * nt, total - 1)];
+#pragma unroll
+            for (int u = 0; u < kStageBatch; ++u) {
+                const int e = base + u * nt;
+                if (e < total) {
+                    const unsigned c = (unsigned)e / hh, t = (unsigned)e - c * hh;
+                    slab[c * Su + t] = v[u];
+                }
+            }
         }
     } else {
-        for (int e = threadIdx.x; e < total; e += blockDim.x) {
-            unsigned c = (unsigned)e / hh, t = (unsigned)e - c * hh;
-            slab[c * S + t] = A[(c0 + c) * lda + t];
+        for (int base = threadIdx.x; base < total; base += kStageBatch * nt) {
+            T v[kStageBatch];
+#pragma unroll
+            for (int u = 0; u < kStageBatch; ++u) {
+                const int e = min(base + u * nt, total - 1);
+                const unsigned c = (unsigned)e / hh, t = (unsigned)e - c * hh;
+                v[u] = A[(c0 + c) * lda + t];
+            }
+#pragma unroll
+            for (int u = 0; u < kStageBatch; ++u) {
+                const int e = base + u * nt;
+                if (e < total) {
+                    const unsigned c = (unsigned)e / hh, t = (unsigned)e - c * hh;
+                    slab[c * Su + t] = v[u];
+                }
+            }
         }
     }
 }
