@@ -165,6 +165,8 @@ gbmv_t_tile(i64 m, i64 n, i64 kl, i64 ku, i64 nrhs, T alpha, const T *__restrict
 
     load_slab(slab, S, A, lda, j0, c1, h);
     const i64 j = j0 + threadIdx.x;
+    // the whole tile is full columns: j0 - ku >= 0, the last column's band ends inside the rows
+    const bool interior = j0 - ku >= 0 && j0 + R <= n && j0 + R - 1 + kl <= m - 1;
     for (i64 r = 0; r < nrhs; ++r) {
         const T *x = X + r * ldx;
         for (i64 e = threadIdx.x; e < W; e += blockDim.x) {
@@ -172,7 +174,30 @@ gbmv_t_tile(i64 m, i64 n, i64 kl, i64 ku, i64 nrhs, T alpha, const T *__restrict
             xs[e] = (row >= 0 && row < m) ? x[row * incx] : T(0);
         }
         __syncthreads();
-        if (j < n) {
+        if (interior) {
+            // every column of the tile has its whole band inside the matrix:
+            // 32-bit indices, eight LDS reads issued before their fmas, the
+            // same ascending order
+            const int jj = threadIdx.x, hi = (int)h, Si = (int)S;
+            const T *ap = slab + jj * Si;  // a_(j-ku+t, j) at ap[t]
+            const T *xp = xs + jj;         // x_(j-ku+t) at xp[t]
+            T acc = T(0);
+            int t = 0;
+            for (; t + 8 <= hi; t += 8) {
+                T av[8], xv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    av[u] = ap[t + u];
+                    xv[u] = xp[t + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) acc = dfma(av[u], xv[u], acc);
+            }
+            for (; t < hi; ++t) acc = dfma(ap[t], xp[t], acc);
+            T *yp = Y + r * ldy + j * incy;
+            T yv = (beta == T(0)) ? T(0) : beta * *yp;
+            *yp = dfma(alpha, acc, yv);
+        } else if (j < n) {
             const i64 ilo = imax(0, j - ku), ihi = imin(m - 1, j + kl);
             T acc = T(0);
             for (i64 ii = ilo; ii <= ihi; ++ii)
