@@ -184,6 +184,18 @@ def bench_bandwidth(bh, steps, skip_cpu=True):
         out[f"gbmv_{tag}_8x8_n2p20"] = {"GBps": nbytes / ms / 1e6, "frac_of_hbm_peak": nbytes / ms / 1e6 / HBM_PEAK_GBPS,
                                         "us": ms * 1e3, "bytes": nbytes}
         if dt == np.float64:
+            # A'x (the transposed product of BandedRows / adjoint views), same rotation
+            def one_t():
+                A, x, y = sets[k[0] % len(sets)]
+                k[0] += 1
+                bh.mul(y, A, x, trans="T")
+
+            for _ in range(6):
+                one_t()
+            bh.sync()
+            ms = gpu_time(bh, one_t, max(30, steps * 6))
+            out["gbmv_fp64_T_8x8_n2p20"] = {"GBps": nbytes / ms / 1e6, "frac_of_hbm_peak": nbytes / ms / 1e6 /
+                                            HBM_PEAK_GBPS, "us": ms * 1e3, "bytes": nbytes}
             # 64 right-hand sides: A read once, X and Y streamed
             X = bh.DeviceArray.from_numpy(np.asfortranarray(rng.standard_normal((n, 64))))
             Y = bh.DeviceArray((n, 64), np.float64)
