@@ -582,7 +582,7 @@ def main():
                     help="run one side workload and print its figures as one JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+        print(f"bench.py: the library runs on one GPU per process (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is a design, not an implementation.", file=sys.stderr)
         sys.exit(2)
 
