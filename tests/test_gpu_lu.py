@@ -24,6 +24,8 @@ LU_CASES = [
     (777, 1, 1, "u", 0.0),
     (500, 0, 5, "n", 0.0),
     (500, 6, 0, "n", 0.0),
+    (600, 24, 24, "n", 0.0),          # kl above 16 with a window under 64 columns: blocked kernel
+    (400, 20, 10, "u", 0.0),
     (300, 40, 24, "n", 0.0),
     (260, 64, 64, "n", 0.0),
 ]
