@@ -37,11 +37,25 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(jobs: int = 8, verbose: bool = False) -> str:
+def _stale(src: str, headers_mtime: float) -> bool:
+    obj = os.path.join(OBJ, os.path.basename(src).replace(".hip", ".o"))
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    return os.path.getmtime(src) > t or headers_mtime > t
+
+
+def build(jobs: int = 8, verbose: bool = False, incremental: bool = False) -> str:
+    """Compile every csrc/*.hip and link the library.  With incremental=True
+    only sources newer than their object (or than any header) are compiled."""
     os.makedirs(OBJ, exist_ok=True)
     srcs = sorted(glob.glob(os.path.join(CSRC, "*.hip")))
+    hdrs = glob.glob(os.path.join(CSRC, "*.hpp")) + glob.glob(os.path.join(HERE, "..", "include", "*.h"))
+    hm = max((os.path.getmtime(h) for h in hdrs), default=0.0)
+    todo = [s for s in srcs if _stale(s, hm)] if incremental else srcs
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(_compile, srcs))
+        list(ex.map(_compile, todo))
+    objs = [os.path.join(OBJ, os.path.basename(s).replace(".hip", ".o")) for s in srcs]
     tmp = OUT + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -49,7 +63,7 @@ def build(jobs: int = 8, verbose: bool = False) -> str:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, OUT)
     if verbose:
-        print(f"built {OUT} from {len(srcs)} sources")
+        print(f"built {OUT} from {len(srcs)} sources ({len(todo)} compiled)")
     return OUT
 
 
@@ -57,4 +71,4 @@ if __name__ == "__main__":
     j = 8
     if "-j" in sys.argv:
         j = int(sys.argv[sys.argv.index("-j") + 1])
-    build(j, verbose=True)
+    build(j, verbose=True, incremental="--incremental" in sys.argv)

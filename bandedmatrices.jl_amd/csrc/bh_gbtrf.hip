@@ -137,12 +137,17 @@ template <typename T>
 int blk_gbtrf_launch(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, i64 *dinfo, hipStream_t st);
 template <typename T>
 int small_gbtrf_launch(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, i64 *dinfo, hipStream_t st);
+template <typename T>
+int mfma_gbtrf_launch(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, i64 *dinfo, hipStream_t st);
 
-// Kernel choice: BANDEDHIP_LU_KERNEL = "small" | "blk" | "reg" | "global"
-// forces one (where it applies).  By default: the one-wave kernel for
-// windows of at most 64 columns (kl + ku <= 63) with kl <= 16, the blocked
-// kernel for kl >= 16 where the one-wave kernel did not apply, the per-step
-// register kernel otherwise.
+// Kernel choice: BANDEDHIP_LU_KERNEL = "small" | "blk" | "reg" | "global" |
+// "mfma" forces one (where it applies).  By default, as LAPACK xGBTRF
+// decides between its unblocked and blocked code (ILAENV block size 32 for
+// ku > 64, blocked when the block size is at most kl): the MFMA-blocked
+// kernel for ku >= 65 and kl >= 32, whose pivots are xGBTRF's and whose
+// factors agree with it to rounding.  Otherwise the xGBTF2-exact kernels:
+// the one-wave kernel for kl <= 16, the register-window blocked kernel for
+// kl >= 16 where that did not apply, the per-step register kernel otherwise.
 static int lu_kernel_override() {
     const char *e = getenv("BANDEDHIP_LU_KERNEL");
     if (!e) return 0;
@@ -150,6 +155,7 @@ static int lu_kernel_override() {
     if (!strcmp(e, "reg")) return 2;
     if (!strcmp(e, "global")) return 3;
     if (!strcmp(e, "small")) return 4;
+    if (!strcmp(e, "mfma")) return 5;
     return 0;
 }
 
@@ -164,15 +170,18 @@ static int gbtrf_impl(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, 
     *info_host = 0;
     if (m == 0 || n == 0) return BH_OK;
     i64 *dinfo = nullptr;
-    BH_HIP(hipMallocAsync((void **)&dinfo, sizeof(i64), stream()));
+    BH_HIP(hipMallocAsync((void **)&dinfo, 2 * sizeof(i64), stream()));
+    BH_HIP(hipMemsetAsync(dinfo, 0, 2 * sizeof(i64), stream()));
     // register-resident kernel when the window fits one of its shapes,
     // otherwise the barrier-stepped kernel working in global memory
     const int ov = lu_kernel_override();
     int st = -1;
+    if (ov == 5 || (ov == 0 && ku >= 65 && kl >= 32))
+        st = mfma_gbtrf_launch<T>(m, n, kl, ku, ab, ldab, ipiv, dinfo, stream());
     // The one-wave lane kernel exists for kl <= 8 and kl = 16; above 16 the
     // blocked kernel beats the generic one-wave kernel (at n = 65536, (24,24):
     // 113.5 ms against 278.6 ms, identical factors), so it goes first there.
-    if (ov == 4 || (ov == 0 && kl <= 16)) st = small_gbtrf_launch<T>(m, n, kl, ku, ab, ldab, ipiv, dinfo, stream());
+    if (st == -1 && (ov == 4 || (ov == 0 && kl <= 16))) st = small_gbtrf_launch<T>(m, n, kl, ku, ab, ldab, ipiv, dinfo, stream());
     if (st == -1 && (ov == 1 || (ov == 0 && kl >= 16)))
         st = blk_gbtrf_launch<T>(m, n, kl, ku, ab, ldab, ipiv, dinfo, stream());
     if (st == -1 && ov != 3) st = reg_gbtrf_launch<T>(m, n, kl, ku, ab, ldab, ipiv, dinfo, stream());
@@ -183,9 +192,15 @@ static int gbtrf_impl(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, 
     } else if (st != BH_OK) {
         return st;
     }
-    BH_HIP(hipMemcpyAsync(info_host, dinfo, sizeof(i64), hipMemcpyDeviceToHost, stream()));
+    i64 hinfo[2] = {0, 0};
+    BH_HIP(hipMemcpyAsync(hinfo, dinfo, 2 * sizeof(i64), hipMemcpyDeviceToHost, stream()));
     BH_HIP(hipFreeAsync(dinfo, stream()));
     BH_HIP(hipStreamSynchronize(stream()));
+    if (hinfo[1] != 0) {
+        set_error("gbtrf: an internal wait of the blocked kernel gave up; the factors are not valid");
+        return BH_ERR_INTERNAL;
+    }
+    *info_host = hinfo[0];
     return BH_OK;
 }
 

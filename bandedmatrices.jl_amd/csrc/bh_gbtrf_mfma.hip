@@ -1,0 +1,593 @@
+// bh_gbtrf_mfma.hip -- blocked partial-pivot band LU with the trailing
+// update on the matrix cores (gfx950 MFMA), FP64 and FP32.
+//
+// This is the counterpart of LAPACK's blocked xGBTRF, the path the reference
+// takes in lu! (LAPACK.gbtrf!, src/banded/BandedLU.jl:98) when ku > 64 and
+// kl >= 32: ILAENV gives a block size of 32 there and xGBTRF leaves xGBTF2.
+// The pivots are chosen exactly as xGBTF2/xGBTRF choose them (first row of
+// the largest |a| among rows j..j+km of the current column, the column
+// having received every earlier update); the factors are the same matrices
+// up to rounding: the updates of a panel reach the trailing band as one
+// rank-16 product on MFMA instead of 16 rank-1 steps, as xGBTRF's DGEMM
+// does.  Output layout, ipiv and info follow LAPACK exactly.
+//
+// One workgroup factors one system.  Window of panel P (j0 = 16P): rows
+// j0 .. j0+R-1 (R = kl + 16) -- every row the panel's pivots and
+// multipliers can touch.  Three kinds of waves:
+//   * the PANEL WAVE factors the 16 panel columns in registers, rows spread
+//     three (PR) to a lane, in xGBTF2's arithmetic (reciprocal-scaled
+//     multipliers, fma rank-1 updates in step order).  Row interchanges only
+//     relabel rows ("slots" keep their data; each slot carries its current
+//     position), so a step moves no data.  The multipliers go to AB at once
+//     in xGBTF2's order; at the end of the panel the slot contents and the
+//     final position -> slot map are published in LDS.
+//   * the HELPER WAVE follows the panel wave one step behind and builds
+//     L11^{-1} (16x16 unit lower) row by row from the pivot rows'
+//     multipliers; it also writes the panel's U11 block to AB.
+//   * eight UPDATE WAVES apply the panel to the trailing block columns (16
+//     columns each) as xGBTRF does, on MFMA 16x16x4:
+//         U12 = L11^{-1} (P A)[0:16]            (4 MFMA)
+//         A22 = (P A)[16:R] - L21 U12           (4 MFMA per 16-row tile)
+//     reading the rows of P A straight from AB through the slot map (a
+//     gather: the permutation costs nothing) and writing U12 and A22 back
+//     in final order.  Block column B is always handled by update wave
+//     B mod 8, so its loads see its own earlier stores.
+// Look-ahead: block column P+1 (the next panel) goes through LDS only.  In
+// the phase where the panel wave factors panel P, the update waves apply
+// panel P-1 to block columns P+1.. (block column P+1 into the LDS
+// look-ahead buffer); after barrier B1 all update waves apply panel P to
+// the look-ahead buffer, one 16-row tile each, into the next panel's input
+// buffer; barrier B2 releases the panel wave onto panel P+1.  All
+// communication between waves is through LDS (barriers order LDS only).
+//
+// Caveats (documented beside the entry point): on finite inputs the pivots
+// equal LAPACK's unless two candidates differ only in rounding; the update
+// sums in a different order than xGBTRF's DGEMM, so factors agree to
+// rounding, not bit for bit.
+#include "bh_common.hpp"
+
+namespace bh {
+
+namespace {
+
+constexpr int QB = 16;       // panel width = block-column width
+constexpr int QUW = 8;       // update waves
+constexpr int QWAVES = QUW + 2;
+constexpr int QBIG = 1 << 29;  // position of a slot that holds no row
+
+template <typename T>
+struct Mf;
+template <>
+struct Mf<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct Mf<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+// Result layout of the 16x16x4 MFMA (checked on gfx950): lane l holds
+// column l%16 of D, element v of its result vector being row drow(l/16, v):
+// 4(l/16)+v for the f32 instruction, l/16+4v for the f64 one.  The operands
+// are A(i, k) from lane i+16k and B(k, j) from lane j+16k for both.
+template <typename T>
+__device__ __forceinline__ int drow(int g, int v) {
+    return sizeof(T) == 4 ? 4 * g + v : g + 4 * v;
+}
+
+template <typename T>
+struct QArgs {
+    T *ab;
+    i64 *ipiv;
+    i64 *info;  // info[0]: LAPACK info; info[1]: nonzero if a wait gave up
+    int m, n, kl, ku;
+    i64 ldab;
+    int R, RT, RP;  // window rows, 16-row tiles, padded rows (16*RT)
+};
+
+template <int K>
+struct QIntC {
+    static constexpr int value = K;
+};
+template <int N>
+struct QFor {
+    template <class F>
+    __device__ __forceinline__ static void run(F &&f) {
+        QFor<N - 1>::run(f);
+        f(QIntC<N - 1>{});
+    }
+};
+template <>
+struct QFor<0> {
+    template <class F>
+    __device__ __forceinline__ static void run(F &&) {}
+};
+
+__device__ __forceinline__ int qmin(int a, int b) { return a < b ? a : b; }
+__device__ __forceinline__ int qmax(int a, int b) { return a > b ? a : b; }
+
+// Pivot-search keys: larger means a better pivot; NaN ranks below every
+// number; 0 marks "not a candidate".
+__device__ __forceinline__ unsigned qkey(float v) {
+    const unsigned b = __builtin_bit_cast(unsigned, v) & 0x7fffffffu;
+    return b > 0x7f800000u ? 1u : b + 2u;
+}
+__device__ __forceinline__ unsigned long long qkey(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v) & 0x7fffffffffffffffull;
+    return b > 0x7ff0000000000000ull ? 1ull : b + 2ull;
+}
+template <int CTRL, int ROWMASK = 0xF>
+__device__ __forceinline__ unsigned qdpp0(unsigned v) {
+    return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWMASK, 0xF, true);
+}
+__device__ __forceinline__ unsigned qumax(unsigned a, unsigned b) { return a > b ? a : b; }
+// maximum over the wave (uniform), DPP with zero fill (0 is below every key)
+__device__ __forceinline__ unsigned wave_max_key(unsigned x) {
+    x = qumax(x, qdpp0<0x111>(x));
+    x = qumax(x, qdpp0<0x112>(x));
+    x = qumax(x, qdpp0<0x114>(x));
+    x = qumax(x, qdpp0<0x118>(x));
+    x = qumax(x, qdpp0<0x142, 0xA>(x));
+    x = qumax(x, qdpp0<0x143, 0xC>(x));
+    return (unsigned)__builtin_amdgcn_readlane((int)x, 63);
+}
+__device__ __forceinline__ unsigned long long wave_max_key(unsigned long long x) {
+    auto step = [](unsigned long long v, unsigned long long o) { return v > o ? v : o; };
+#define QDPP64(CTRL, RM)                                                                              \
+    {                                                                                                 \
+        const unsigned lo = qdpp0<CTRL, RM>((unsigned)x), hi = qdpp0<CTRL, RM>((unsigned)(x >> 32)); \
+        x = step(x, ((unsigned long long)hi << 32) | lo);                                             \
+    }
+    QDPP64(0x111, 0xF)
+    QDPP64(0x112, 0xF)
+    QDPP64(0x114, 0xF)
+    QDPP64(0x118, 0xF)
+    QDPP64(0x142, 0xA)
+    QDPP64(0x143, 0xC)
+#undef QDPP64
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)x, 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), 63);
+    return ((unsigned long long)hi << 32) | lo;
+}
+// minimum of non-negative ints over the wave (uniform)
+__device__ __forceinline__ int wave_min_int(int v) {
+    // max of the complement
+    unsigned x = 0x7fffffffu - (unsigned)v;
+    x = wave_max_key(x);
+    return (int)(0x7fffffffu - x);
+}
+
+__device__ __forceinline__ int lds_load_relaxed(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_store_relaxed(int *p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <typename T, int PR>
+__global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
+    using v4 = typename Mf<T>::v4;
+    const int m = s.m, n = s.n, kl = s.kl, ku = s.ku, kv = kl + ku;
+    const int R = s.R, RT = s.RT, RP = s.RP;
+    const i64 ld = s.ldab;
+    T *__restrict__ ab = s.ab;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int mn = qmin(m, n);
+    const int NP = (mn + QB - 1) / QB;
+
+    // ---- LDS
+    extern __shared__ __align__(16) unsigned char q_raw[];
+    T *pbuf = reinterpret_cast<T *>(q_raw);  // [2][RP][QB] panel slots after factoring
+    T *pan = pbuf + 2 * RP * QB;              // [2][RP][QB] next panel input (position order)
+    T *lab = pan + 2 * RP * QB;               // [RP][QB]    look-ahead block column (slot order)
+    T *colb = lab + RP * QB;                  // [QB][RP]    negated multipliers per step
+    T *linv = colb + QB * RP;                 // [2][QB][QB] L11^{-1}, row major
+    __shared__ int s_src[2][256];             // final position -> slot
+    __shared__ int s_sig[QB];                 // pivot slot of each step
+    __shared__ int s_ju[2];
+    __shared__ int s_step;
+    __shared__ int s_info;
+    __shared__ int s_err;
+
+    if (mn == 0) {
+        if (tid == 0) {
+            s.info[0] = 0;
+            s.info[1] = 0;
+        }
+        return;
+    }
+
+    // entry (r, c) of AB as it stands (0 outside the matrix or the stored band)
+    auto gload = [&](int r, int c) -> T {
+        if (r < 0 || r >= m || c >= n || c - r > kv || r - c > kl) return T(0);
+        return ab[(i64)c * ld + (kv + r - c)];
+    };
+    auto gstore = [&](int r, int c, T v) {
+        if (r >= 0 && r < m && c < n && c - r <= kv && r - c <= kl) ab[(i64)c * ld + (kv + r - c)] = v;
+    };
+
+    // ---- prologue: panel 0 input, look-ahead buffer of panel 0, next panel's new rows
+    if (tid == 0) {
+        s_step = 0;
+        s_info = 0;
+        s_err = 0;
+    }
+    if (wv >= 2) {
+        const int t0 = tid - 128, nt = 64 * QUW;
+        for (int e = t0; e < RP * QB; e += nt) {
+            const int q = e / QB, c = e - q * QB;
+            pan[e] = q < R ? gload(q, c) : T(0);
+            lab[e] = q < R ? gload(q, QB + c) : T(0);
+        }
+        for (int e = t0; e < QB * QB; e += nt) {
+            const int i = e / QB, c = e - i * QB;
+            pan[RP * QB + (R - QB + i) * QB + c] = gload(R + i, QB + c);
+        }
+    }
+    lds_barrier();  // B0
+
+    if (wv == 0) {
+        // ================================================================ panel wave
+        __builtin_amdgcn_s_setprio(3);
+        int ju = 0, info = 0;
+        for (int P = 0; P < NP; ++P) {
+            const int j0 = P * QB, np = qmin(QB, mn - j0), b = P & 1;
+            const T *pin = pan + b * RP * QB;
+            T pp[PR][QB];
+            int pos[PR];
+#pragma unroll
+            for (int r = 0; r < PR; ++r) {
+                const int q = lane + 64 * r;
+                pos[r] = (q < R && j0 + q < m) ? q : QBIG;
+#pragma unroll
+                for (int c = 0; c < QB; ++c) pp[r][c] = (q < R) ? pin[q * QB + c] : T(0);
+            }
+            int myp = 0;
+            QFor<QB>::run([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                if (k < np) {
+                    const int j = j0 + k;
+                    const int km = qmin(kl, m - 1 - j);
+                    // ---- pivot: largest |a| over positions k..k+km, smallest position on ties
+                    int bp = QBIG, br = 0;
+                    bool top;
+                    if constexpr (sizeof(T) == 4) {
+                        unsigned bk = 0;
+#pragma unroll
+                        for (int r = 0; r < PR; ++r) {
+                            const int L = pos[r];
+                            const unsigned key = (L >= k && L <= k + km) ? qkey(pp[r][k]) : 0u;
+                            const bool better = key > bk || (key == bk && L < bp);
+                            bk = better ? key : bk;
+                            bp = better ? L : bp;
+                            br = better ? r : br;
+                        }
+                        const auto mx = wave_max_key(bk);  // all lanes take part in the reduction
+                        top = (bk != 0u) && (bk == mx);
+                    } else {
+                        unsigned long long bk = 0;
+#pragma unroll
+                        for (int r = 0; r < PR; ++r) {
+                            const int L = pos[r];
+                            const unsigned long long key = (L >= k && L <= k + km) ? qkey(pp[r][k]) : 0ull;
+                            const bool better = key > bk || (key == bk && L < bp);
+                            bk = better ? key : bk;
+                            bp = better ? L : bp;
+                            br = better ? r : br;
+                        }
+                        const auto mx = wave_max_key(bk);  // all lanes take part in the reduction
+                        top = (bk != 0ull) && (bk == mx);
+                    }
+                    unsigned long long tops = (unsigned long long)__ballot(top);
+                    int lp;
+                    if (__popcll(tops) == 1) {
+                        lp = __builtin_amdgcn_readfirstlane((int)__ffsll((long long)tops) - 1);
+                    } else {
+                        const int pmin = wave_min_int(top ? bp : QBIG);
+                        tops = (unsigned long long)__ballot(top && bp == pmin);
+                        lp = __builtin_amdgcn_readfirstlane((int)__ffsll((long long)tops) - 1);
+                    }
+                    const int p = __builtin_amdgcn_readlane(bp, lp);
+                    const int rp = __builtin_amdgcn_readlane(br, lp);
+                    // ---- the pivot row, columns k .. QB-1 (uniform)
+                    T u[QB];
+#pragma unroll
+                    for (int c = 0; c < QB; ++c) u[c] = T(0);
+#pragma unroll
+                    for (int r = 0; r < PR; ++r) {
+                        if (rp == r) {
+#pragma unroll
+                            for (int c = k; c < QB; ++c) u[c] = lane_read(pp[r][c], lp);
+                        }
+                    }
+                    const T pv = u[k];
+                    T *col = ab + (i64)j * ld + (kv - k);  // col[L]: row j0+L of column j
+                    T *cb = colb + k * RP;
+                    if (pv != T(0)) {
+                        ju = qmax(ju, qmin(j + ku + (p - k), n - 1));
+                        // interchange = relabel: the slot at position k takes position p,
+                        // the pivot slot takes position k
+#pragma unroll
+                        for (int r = 0; r < PR; ++r) {
+                            pos[r] = (pos[r] == k) ? p : pos[r];
+                            pos[r] = (r == rp && lane == lp) ? k : pos[r];
+                        }
+                        const T rcp = T(1) / pv;
+#pragma unroll
+                        for (int r = 0; r < PR; ++r) {
+                            const int L = pos[r];
+                            const int q = lane + 64 * r;
+                            const bool act = (L > k) && (L <= k + km);
+                            const T l = pp[r][k] * rcp;
+                            if (act) {
+                                col[L] = l;
+                                pp[r][k] = l;
+#pragma unroll
+                                for (int c = k + 1; c < QB; ++c) pp[r][c] = dfma(l, -u[c], pp[r][c]);
+                            }
+                            if (q < RP) cb[q] = act ? -l : T(0);
+                        }
+                    } else {
+                        if (info == 0) info = j + 1;
+#pragma unroll
+                        for (int r = 0; r < PR; ++r) {
+                            const int L = pos[r];
+                            const int q = lane + 64 * r;
+                            const bool act = (L > k) && (L <= k + km);
+                            if (act) col[L] = pp[r][k];
+                            if (q < RP) cb[q] = T(0);
+                        }
+                    }
+                    if (lane == k) myp = p;
+                    if (lane == 0) s_sig[k] = lp + 64 * rp;
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+                    if (lane == 0) lds_store_relaxed(&s_step, P * QB + k + 1);
+                }
+            });
+            // ---- publish: pivots, slot contents, final position -> slot
+            if (lane < np) s.ipiv[j0 + lane] = (i64)(j0 + myp + 1);
+            T *pb = pbuf + b * RP * QB;
+#pragma unroll
+            for (int r = 0; r < PR; ++r) {
+                const int q = lane + 64 * r;
+                if (q < RP) {
+#pragma unroll
+                    for (int c = 0; c < QB; ++c) pb[q * QB + c] = pp[r][c];
+                    const int f = pos[r] < QBIG ? pos[r] : q;
+                    s_src[b][f] = q;
+                }
+            }
+            if (lane == 0) {
+                s_ju[b] = ju;
+                s_info = info;
+            }
+            lds_barrier();  // B1
+            lds_barrier();  // B2
+        }
+    } else if (wv == 1) {
+        // ================================================================ helper wave
+        const int c16 = lane & 15;
+        auto write_u11 = [&](int Pp) {
+            // U11 of panel Pp: U[k][c] = row of the pivot slot of step k, column c >= k
+            const int j0 = Pp * QB, np = qmin(QB, mn - j0), b = Pp & 1;
+            const T *pb = pbuf + b * RP * QB;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int k = (lane >> 4) + 4 * i;
+                if (k < np && k <= c16) {
+                    const int q = s_src[b][k];
+                    gstore(j0 + k, j0 + c16, pb[q * QB + c16]);
+                }
+            }
+        };
+        for (int P = 0; P < NP; ++P) {
+            const int j0 = P * QB, np = qmin(QB, mn - j0), b = P & 1;
+            // X = L11^{-1}; lane j (< 16) holds column j, X[i] = X(i, j)
+            T X[QB];
+#pragma unroll
+            for (int i = 0; i < QB; ++i) X[i] = (i == lane) ? T(1) : T(0);
+            QFor<QB>::run([&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                if (k < np) {
+                    const int target = P * QB + k + 1;
+                    int it = 0;
+                    while (lds_load_relaxed(&s_step) < target) {
+                        if (++it > (1 << 22)) {
+                            s_err = 1;
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+                    const int q = s_sig[k];
+                    // row k: X(k, j) = [k == j] - sum_{k' < k} L11(k, k') X(k', j); colb holds -l
+                    T x = (lane == k) ? T(1) : T(0);
+#pragma unroll
+                    for (int kk = 0; kk < k; ++kk) x = dfma(colb[kk * RP + q], X[kk], x);
+                    X[k] = x;
+                }
+            });
+            if (lane < QB) {
+                T *li = linv + b * QB * QB;
+#pragma unroll
+                for (int i = 0; i < QB; ++i) li[i * QB + lane] = X[i];
+            }
+            if (P >= 1) write_u11(P - 1);
+            lds_barrier();  // B1
+            lds_barrier();  // B2
+        }
+        write_u11(NP - 1);
+        if (lane == 0) {
+            // the last wave to finish the panel loop reports
+            s.info[0] = (i64)s_info;
+            s.info[1] = (i64)s_err;
+        }
+    } else {
+        // ================================================================ update waves
+        const int w = wv - 2;
+        const int g = lane >> 4, li = lane & 15;
+        // Apply panel Pp to block column B.  mode 0: AB -> AB; mode 1: AB -> look-ahead
+        // buffer (next panel's positions); mode 2: look-ahead buffer -> next panel input,
+        // tiles rt0, rt0+QUW, ... only, U rows stored when store_u.
+        auto update_bc = [&](int Pp, int B, int mode, int rt0, int rstep, bool store_u) {
+            const int j0p = Pp * QB, npp = qmin(QB, mn - j0p), b = Pp & 1;
+            const int c0 = B * QB;
+            const int *src = s_src[b];
+            const T *pb = pbuf + b * RP * QB;
+            // (P A)[0:16] as the B operand: rows f = 4g+i, column c0+li
+            T braw[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int f = 4 * g + i;
+                const int q = src[f];
+                T v = T(0);
+                if (f < npp) v = (mode == 2) ? lab[q * QB + li] : gload(j0p + q, c0 + li);
+                braw[i] = v;
+            }
+            const T *lv = linv + b * QB * QB + li * QB + 4 * g;
+            v4 u = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+            for (int i = 0; i < 4; ++i) u = Mf<T>::mma(lv[i], braw[i], u);
+            T *dstp = (mode == 2) ? pan + ((Pp + 1) & 1) * RP * QB : lab;
+            for (int rt = rt0; rt < RT; rt += rstep) {
+                const int f = 16 * rt + li;  // final position of this lane's row
+                const int q = src[f];
+                // acc[v]: row f, column c0 + drow(g,v) (the transposed product's result)
+                v4 acc;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int c = drow<T>(g, v);
+                    acc[v] = (mode == 2) ? lab[q * QB + c] : gload(j0p + q, c0 + c);
+                }
+                // step i: k = drow(g, i) on both operands (u[i] is U12(drow(g,i), .))
+                const T *lr = pb + q * QB;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) acc = Mf<T>::mma(u[i], -lr[drow<T>(g, i)], acc);
+                if (f < R) {
+                    if (mode == 0) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) gstore(j0p + f, c0 + drow<T>(g, v), acc[v]);
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) dstp[(f - QB) * QB + drow<T>(g, v)] = acc[v];
+                    }
+                }
+            }
+            // u[v] = U12(drow(g,v), c0+li).  Stored after every tile has read its
+            // rows: the rows displaced by this panel's pivots are read from the
+            // positions the U rows go to.
+            if (store_u) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int f = drow<T>(g, v);
+                    if (f < npp) gstore(j0p + f, c0 + li, u[v]);
+                }
+            }
+        };
+        for (int P = 0; P <= NP; ++P) {
+            // ---- phase 1: panel P-1 onto block columns P+1 ..
+            if (P >= 1) {
+                const int Pp = P - 1;
+                const int jup = s_ju[Pp & 1];
+                const int Bmax = qmin((n - 1) / QB, jup / QB);
+                const bool la = (P < NP) && ((P + 1) * QB < n);
+                const int Bend = la ? qmax(Bmax, P + 1) : Bmax;
+                for (int B = P + 1; B <= Bend; ++B) {
+                    if (B % QUW != w) continue;
+                    const bool tola = la && B == P + 1;
+                    update_bc(Pp, B, tola ? 1 : 0, 1, 1, true);
+                    if (tola) {
+                        // rows entering: look-ahead buffer positions R-16.. and the
+                        // next panel's input positions R-16.. (untouched original rows)
+                        const int j0 = P * QB, c0 = B * QB;
+                        T *pn = pan + ((P + 1) & 1) * RP * QB;
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) {
+                            const int e = lane + 64 * i, r = e >> 4, c = e & 15;
+                            lab[(R - QB + r) * QB + c] = gload(j0 + R - QB + r, c0 + c);
+                            pn[(R - QB + r) * QB + c] = gload(j0 + R + r, c0 + c);
+                        }
+                    }
+                }
+            }
+            if (P == NP) break;
+            lds_barrier();  // B1
+            // ---- phase 2: panel P onto the look-ahead block column P+1
+            if ((P + 1) * QB < n) update_bc(P, P + 1, 2, 1 + w, QUW, w == 0);
+            lds_barrier();  // B2
+        }
+    }
+}
+
+template <typename T, int PR>
+int mfma_launch_one(const QArgs<T> &a, hipStream_t st) {
+    const size_t lds = sizeof(T) * ((size_t)6 * a.RP * QB + 2 * QB * QB);
+    auto kern = gbtrf_mfma_kernel<T, PR>;
+    if (lds > 64 * 1024)
+        BH_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kern, dim3(1), dim3(64 * QWAVES), lds, st, a);
+    BH_LAUNCH_CHECK();
+    return BH_OK;
+}
+
+// LAPACK zeroes the fill-in rows of columns ku+1 .. kv-1 before the first
+// step and those of column j+kv at step j (j < min(m,n)); zero them all up
+// front (rows whose row index is >= 0 only).
+template <typename T>
+__global__ void gbtrf_zero_fill(T *ab, i64 ld, int n, int mn, int kl, int ku) {
+    const int kv = kl + ku;
+    const int c = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int cend = qmin(n, mn + kv);
+    if (c <= ku || c >= cend) return;
+    for (int r = qmax(0, kv - c) + (threadIdx.x & 63); r < kl; r += 64) ab[(i64)c * ld + r] = T(0);
+}
+
+}  // namespace
+
+// LDS bytes the kernel needs for a band, or 0 when it does not apply.
+size_t mfma_lds_bytes(i64 kl, i64 elem) {
+    if (kl < 1 || kl > 240) return 0;
+    const i64 R = kl + QB, RP = 16 * ((R + 15) / 16);
+    return (size_t)elem * ((size_t)6 * RP * QB + 2 * QB * QB);
+}
+
+// The blocked MFMA band LU, or -1 when the band does not fit it (kl + 16 rows
+// in at most four slots per lane and the LDS buffers within 160 KiB).
+template <typename T>
+int mfma_gbtrf_launch(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, i64 *dinfo, hipStream_t st) {
+    const size_t lds = mfma_lds_bytes(kl, sizeof(T));
+    if (lds == 0 || lds > 160 * 1024 || m >= (1ll << 29) || n >= (1ll << 29) || kl + ku >= (1 << 20)) return -1;
+    const int R = (int)kl + QB, RT = (R + 15) / 16, RP = 16 * RT;
+    QArgs<T> a{ab, ipiv, dinfo, (int)m, (int)n, (int)kl, (int)ku, ldab, R, RT, RP};
+    const int mn = (int)imin(m, n);
+    const int cend = (int)imin(n, (i64)mn + kl + ku);
+    if (cend > ku + 1) {
+        const int per = 4;  // columns per 256-thread block
+        hipLaunchKernelGGL(gbtrf_zero_fill<T>, dim3((cend + per - 1) / per), dim3(64 * per), 0, st, ab, ldab, (int)n,
+                           mn, (int)kl, (int)ku);
+        BH_LAUNCH_CHECK();
+    }
+    const int PR = (R + 63) / 64;
+    switch (PR) {
+        case 1:
+            return mfma_launch_one<T, 1>(a, st);
+        case 2:
+            return mfma_launch_one<T, 2>(a, st);
+        case 3:
+            return mfma_launch_one<T, 3>(a, st);
+        case 4:
+            return mfma_launch_one<T, 4>(a, st);
+        default:
+            return -1;
+    }
+}
+
+template int mfma_gbtrf_launch<double>(i64, i64, i64, i64, double *, i64, i64 *, i64 *, hipStream_t);
+template int mfma_gbtrf_launch<float>(i64, i64, i64, i64, float *, i64, i64 *, i64 *, hipStream_t);
+
+}  // namespace bh

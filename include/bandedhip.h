@@ -36,6 +36,7 @@ typedef int64_t bh_int;
 #define BH_ERR_NODEVICE (-1002)   /* no usable gfx950 device */
 #define BH_ERR_BANDERROR (-1003)  /* C lacks bands for A*B (BandError, Band.jl:75-88) */
 #define BH_ERR_WORKSPACE (-1004)  /* workspace too small */
+#define BH_ERR_INTERNAL (-1005)   /* a kernel's bounded internal wait gave up (results invalid) */
 
 /* ---------------------------------------------------------------- runtime */
 /* No reference counterpart: device/memory plumbing the Julia extension uses to
