@@ -80,6 +80,12 @@ __device__ __forceinline__ int drow(int g, int v) {
     return sizeof(T) == 4 ? 4 * g + v : g + 4 * v;
 }
 
+// LDS row stride (elements) of the [rows][16] buffers: 80 and 144 bytes
+template <typename T>
+struct QStride {
+    static constexpr int value = sizeof(T) == 4 ? 20 : 18;
+};
+
 template <typename T>
 struct QArgs {
     T *ab;
@@ -154,12 +160,86 @@ __device__ __forceinline__ unsigned long long wave_max_key(unsigned long long x)
     const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(x >> 32), 63);
     return ((unsigned long long)hi << 32) | lo;
 }
+// High word of |v| as an unsigned integer, monotone in |v| over numbers and
+// infinities; ~0u for a NaN.
+__device__ __forceinline__ unsigned qhiword(float v) {
+    const unsigned b = __builtin_bit_cast(unsigned, v) & 0x7fffffffu;
+    return b > 0x7f800000u ? 0xffffffffu : b;
+}
+__device__ __forceinline__ unsigned qhiword(double v) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v) & 0x7fffffffffffffffull;
+    return b > 0x7ff0000000000000ull ? 0xffffffffu : (unsigned)(b >> 32);
+}
+// Reciprocal of a nonzero pivot: the hardware estimate refined by Newton
+// steps (error within an ulp or so; the blocked factorisation promises
+// agreement with LAPACK to rounding only).
+__device__ __forceinline__ float qrcp(float x) {
+    float r = __builtin_amdgcn_rcpf(x);
+    const float e = __builtin_fmaf(-x, r, 1.0f);
+    return __builtin_fmaf(r, e, r);
+}
+__device__ __forceinline__ double qrcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+// maximum over the wave (uniform) of non-NaN values; lanes without a source
+// keep their own value
+__device__ __forceinline__ float wave_fmax(float x) {
+    x = __builtin_fmaxf(x, dpp_mov<0x111>(x, x));
+    x = __builtin_fmaxf(x, dpp_mov<0x112>(x, x));
+    x = __builtin_fmaxf(x, dpp_mov<0x114>(x, x));
+    x = __builtin_fmaxf(x, dpp_mov<0x118>(x, x));
+    x = __builtin_fmaxf(x, dpp_mov<0x142, 0xA>(x, x));
+    x = __builtin_fmaxf(x, dpp_mov<0x143, 0xC>(x, x));
+    return lane_read(x, 63);
+}
+__device__ __forceinline__ double wave_fmax(double x) {
+    x = __builtin_fmax(x, dpp_mov<0x111>(x, x));
+    x = __builtin_fmax(x, dpp_mov<0x112>(x, x));
+    x = __builtin_fmax(x, dpp_mov<0x114>(x, x));
+    x = __builtin_fmax(x, dpp_mov<0x118>(x, x));
+    x = __builtin_fmax(x, dpp_mov<0x142, 0xA>(x, x));
+    x = __builtin_fmax(x, dpp_mov<0x143, 0xC>(x, x));
+    return lane_read(x, 63);
+}
 // minimum of non-negative ints over the wave (uniform)
 __device__ __forceinline__ int wave_min_int(int v) {
     // max of the complement
     unsigned x = 0x7fffffffu - (unsigned)v;
     x = wave_max_key(x);
     return (int)(0x7fffffffu - x);
+}
+
+// Raw buffer resource (word 3 for gfx9-family parts) and the offset that
+// lies beyond any resource: loads there return 0, stores there are dropped.
+constexpr int kQRsrcWord3 = 0x00020000;
+constexpr int kQOob = 0x7ffffff0;
+template <typename T>
+__device__ __forceinline__ T qbload(__amdgpu_buffer_rsrc_t r, int off);
+template <>
+__device__ __forceinline__ float qbload<float>(__amdgpu_buffer_rsrc_t r, int off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <>
+__device__ __forceinline__ double qbload<double>(__amdgpu_buffer_rsrc_t r, int off) {
+    typedef unsigned v2u __attribute__((ext_vector_type(2)));
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+template <typename T>
+__device__ __forceinline__ void qbstore(__amdgpu_buffer_rsrc_t r, int off, T v);
+template <>
+__device__ __forceinline__ void qbstore<float>(__amdgpu_buffer_rsrc_t r, int off, float v) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+}
+template <>
+__device__ __forceinline__ void qbstore<double>(__amdgpu_buffer_rsrc_t r, int off, double v) {
+    typedef unsigned v2u __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, v), r, off, 0, 0);
 }
 
 __device__ __forceinline__ int lds_load_relaxed(const int *p) {
@@ -172,9 +252,11 @@ __device__ __forceinline__ void lds_store_relaxed(int *p, int v) {
 template <typename T, int PR>
 __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
     using v4 = typename Mf<T>::v4;
+    constexpr int QS = QStride<T>::value;
     const int m = s.m, n = s.n, kl = s.kl, ku = s.ku, kv = kl + ku;
     const int R = s.R, RT = s.RT, RP = s.RP;
     const i64 ld = s.ldab;
+    const int ldi = (int)ld;  // 16 columns of AB fit 2^31 bytes (checked at launch)
     T *__restrict__ ab = s.ab;
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int mn = qmin(m, n);
@@ -182,11 +264,13 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
 
     // ---- LDS
     extern __shared__ __align__(16) unsigned char q_raw[];
-    T *pbuf = reinterpret_cast<T *>(q_raw);  // [2][RP][QB] panel slots after factoring
-    T *pan = pbuf + 2 * RP * QB;              // [2][RP][QB] next panel input (position order)
-    T *lab = pan + 2 * RP * QB;               // [RP][QB]    look-ahead block column (slot order)
-    T *colb = lab + RP * QB;                  // [QB][RP]    negated multipliers per step
-    T *linv = colb + QB * RP;                 // [2][QB][QB] L11^{-1}, row major
+    // rows of 16 entries at a padded stride QS (no LDS bank conflicts when the
+    // 16 rows of a tile are read column by column)
+    T *pbuf = reinterpret_cast<T *>(q_raw);  // [2][RP][QS] panel slots after factoring
+    T *pan = pbuf + 2 * RP * QS;              // [2][RP][QS] next panel input (position order)
+    T *lab = pan + 2 * RP * QS;               // [RP][QS]    look-ahead block column (slot order)
+    T *colb = lab + RP * QS;                  // [QB][64*PR] negated multipliers per step
+    T *linv = colb + QB * 64 * PR;            // [2][QB][QB] L11^{-1}, row major
     __shared__ int s_src[2][256];             // final position -> slot
     __shared__ int s_sig[QB];                 // pivot slot of each step
     __shared__ int s_ju[2];
@@ -221,12 +305,12 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
         const int t0 = tid - 128, nt = 64 * QUW;
         for (int e = t0; e < RP * QB; e += nt) {
             const int q = e / QB, c = e - q * QB;
-            pan[e] = q < R ? gload(q, c) : T(0);
-            lab[e] = q < R ? gload(q, QB + c) : T(0);
+            pan[q * QS + c] = q < R ? gload(q, c) : T(0);
+            lab[q * QS + c] = q < R ? gload(q, QB + c) : T(0);
         }
         for (int e = t0; e < QB * QB; e += nt) {
             const int i = e / QB, c = e - i * QB;
-            pan[RP * QB + (R - QB + i) * QB + c] = gload(R + i, QB + c);
+            pan[RP * QS + (R - QB + i) * QS + c] = gload(R + i, QB + c);
         }
     }
     lds_barrier();  // B0
@@ -237,7 +321,7 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
         int ju = 0, info = 0;
         for (int P = 0; P < NP; ++P) {
             const int j0 = P * QB, np = qmin(QB, mn - j0), b = P & 1;
-            const T *pin = pan + b * RP * QB;
+            const T *pin = pan + b * RP * QS;
             T pp[PR][QB];
             int pos[PR];
 #pragma unroll
@@ -245,55 +329,80 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
                 const int q = lane + 64 * r;
                 pos[r] = (q < R && j0 + q < m) ? q : QBIG;
 #pragma unroll
-                for (int c = 0; c < QB; ++c) pp[r][c] = (q < R) ? pin[q * QB + c] : T(0);
+                for (int c = 0; c < QB; ++c) pp[r][c] = (q < R) ? pin[q * QS + c] : T(0);
             }
             int myp = 0;
+            // multipliers of a step go to AB column j through a buffer resource over
+            // the panel's columns (an inactive slot's store lands out of range)
+            const int ncol = qmin(QB, n - j0);
+            const __amdgpu_buffer_rsrc_t rpan = __builtin_amdgcn_make_buffer_rsrc(
+                ab + (i64)j0 * ld, 0, (int)((i64)ncol * ld * (i64)sizeof(T)), kQRsrcWord3);
             QFor<QB>::run([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
                 if (k < np) {
                     const int j = j0 + k;
                     const int km = qmin(kl, m - 1 - j);
-                    // ---- pivot: largest |a| over positions k..k+km, smallest position on ties
-                    int bp = QBIG, br = 0;
-                    bool top;
-                    if constexpr (sizeof(T) == 4) {
-                        unsigned bk = 0;
+                    // ---- pivot: largest |a| over positions k..k+km, smallest position
+                    // on ties.  Fast path: integer keys from the magnitude's high word,
+                    // one DPP maximum, one ballot per slot; when more than one slot holds
+                    // the maximum key (a tie, equal high words, NaN, an all-zero column)
+                    // the exact search decides.
+                    unsigned key[PR];
+                    unsigned kmax = 0;
 #pragma unroll
-                        for (int r = 0; r < PR; ++r) {
-                            const int L = pos[r];
-                            const unsigned key = (L >= k && L <= k + km) ? qkey(pp[r][k]) : 0u;
-                            const bool better = key > bk || (key == bk && L < bp);
-                            bk = better ? key : bk;
-                            bp = better ? L : bp;
-                            br = better ? r : br;
+                    for (int r = 0; r < PR; ++r) {
+                        const bool cand = (unsigned)(pos[r] - k) <= (unsigned)km;
+                        const unsigned hw = qhiword(pp[r][k]);
+                        key[r] = (cand && hw != 0xffffffffu) ? hw + 1u : 0u;
+                        kmax = qumax(kmax, key[r]);
+                    }
+                    const unsigned M = wave_max_key(kmax);
+                    unsigned long long tm[PR];
+                    int ntop = 0;
+#pragma unroll
+                    for (int r = 0; r < PR; ++r) {
+                        tm[r] = (unsigned long long)__ballot(key[r] == M);
+                        ntop += __popcll(tm[r]);
+                    }
+                    int lp, p, rp;
+                    if (ntop == 1 && M > 1u) {
+                        rp = 0;
+                        unsigned long long t = tm[0];
+#pragma unroll
+                        for (int r = 1; r < PR; ++r) {
+                            rp = tm[r] ? r : rp;
+                            t = tm[r] ? tm[r] : t;
                         }
-                        const auto mx = wave_max_key(bk);  // all lanes take part in the reduction
-                        top = (bk != 0u) && (bk == mx);
+                        lp = (int)__ffsll((long long)t) - 1;
+                        p = __builtin_amdgcn_readlane(pos[0], lp);
+#pragma unroll
+                        for (int r = 1; r < PR; ++r) {
+                            const int pr_ = __builtin_amdgcn_readlane(pos[r], lp);
+                            p = (rp == r) ? pr_ : p;
+                        }
                     } else {
+                        // exact: keys over the full value (NaN below every number), the
+                        // smallest position first
                         unsigned long long bk = 0;
+                        int kp = QBIG, kr = 0;
 #pragma unroll
                         for (int r = 0; r < PR; ++r) {
                             const int L = pos[r];
-                            const unsigned long long key = (L >= k && L <= k + km) ? qkey(pp[r][k]) : 0ull;
-                            const bool better = key > bk || (key == bk && L < bp);
-                            bk = better ? key : bk;
-                            bp = better ? L : bp;
-                            br = better ? r : br;
+                            const unsigned long long kk =
+                                (L >= k && L <= k + km) ? (unsigned long long)qkey(pp[r][k]) : 0ull;
+                            const bool better = kk > bk || (kk == bk && L < kp);
+                            bk = better ? kk : bk;
+                            kp = better ? L : kp;
+                            kr = better ? r : kr;
                         }
-                        const auto mx = wave_max_key(bk);  // all lanes take part in the reduction
-                        top = (bk != 0ull) && (bk == mx);
+                        const unsigned long long mk = wave_max_key(bk);
+                        const bool tk = (bk != 0ull) && (bk == mk);
+                        const int pmin = wave_min_int(tk ? kp : QBIG);
+                        const unsigned long long t2 = (unsigned long long)__ballot(tk && kp == pmin);
+                        lp = __builtin_amdgcn_readfirstlane((int)__ffsll((long long)t2) - 1);
+                        p = __builtin_amdgcn_readlane(kp, lp);
+                        rp = __builtin_amdgcn_readlane(kr, lp);
                     }
-                    unsigned long long tops = (unsigned long long)__ballot(top);
-                    int lp;
-                    if (__popcll(tops) == 1) {
-                        lp = __builtin_amdgcn_readfirstlane((int)__ffsll((long long)tops) - 1);
-                    } else {
-                        const int pmin = wave_min_int(top ? bp : QBIG);
-                        tops = (unsigned long long)__ballot(top && bp == pmin);
-                        lp = __builtin_amdgcn_readfirstlane((int)__ffsll((long long)tops) - 1);
-                    }
-                    const int p = __builtin_amdgcn_readlane(bp, lp);
-                    const int rp = __builtin_amdgcn_readlane(br, lp);
                     // ---- the pivot row, columns k .. QB-1 (uniform)
                     T u[QB];
 #pragma unroll
@@ -306,9 +415,8 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
                         }
                     }
                     const T pv = u[k];
-                    T *col = ab + (i64)j * ld + (kv - k);  // col[L]: row j0+L of column j
-                    T *cb = colb + k * RP;
-                    if (pv != T(0)) {
+                    const bool nz = (pv != T(0));
+                    if (nz) {
                         ju = qmax(ju, qmin(j + ku + (p - k), n - 1));
                         // interchange = relabel: the slot at position k takes position p,
                         // the pivot slot takes position k
@@ -317,47 +425,46 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
                             pos[r] = (pos[r] == k) ? p : pos[r];
                             pos[r] = (r == rp && lane == lp) ? k : pos[r];
                         }
-                        const T rcp = T(1) / pv;
+                    } else if (info == 0) {
+                        info = j + 1;
+                    }
+                    const T rcp = nz ? qrcp(pv) : T(1);
+                    T *cb = colb + k * (64 * PR);
 #pragma unroll
-                        for (int r = 0; r < PR; ++r) {
-                            const int L = pos[r];
-                            const int q = lane + 64 * r;
-                            const bool act = (L > k) && (L <= k + km);
-                            const T l = pp[r][k] * rcp;
-                            if (act) {
-                                col[L] = l;
-                                pp[r][k] = l;
+                    for (int r = 0; r < PR; ++r) {
+                        const int L = pos[r];
+                        const bool act = (L > k) && (L <= k + km);
+                        const T raw = pp[r][k];
+                        const T l = nz ? raw * rcp : raw;  // a zero pivot leaves the column as it is
+                        const int off = act ? (k * (ldi - 1) + kv + L) * (int)sizeof(T) : kQOob;
+                        qbstore<T>(rpan, off, l);
+                        pp[r][k] = act ? l : raw;
+                        const T le = (act && nz) ? l : T(0);
+                        cb[lane + 64 * r] = -le;
+                        // rank-1 update of the panel's later columns; inactive slots take
+                        // multiplier 0 (a signed zero can lose its sign there)
 #pragma unroll
-                                for (int c = k + 1; c < QB; ++c) pp[r][c] = dfma(l, -u[c], pp[r][c]);
-                            }
-                            if (q < RP) cb[q] = act ? -l : T(0);
-                        }
-                    } else {
-                        if (info == 0) info = j + 1;
-#pragma unroll
-                        for (int r = 0; r < PR; ++r) {
-                            const int L = pos[r];
-                            const int q = lane + 64 * r;
-                            const bool act = (L > k) && (L <= k + km);
-                            if (act) col[L] = pp[r][k];
-                            if (q < RP) cb[q] = T(0);
-                        }
+                        for (int c = k + 1; c < QB; ++c) pp[r][c] = dfma(le, -u[c], pp[r][c]);
                     }
                     if (lane == k) myp = p;
-                    if (lane == 0) s_sig[k] = lp + 64 * rp;
-                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-                    if (lane == 0) lds_store_relaxed(&s_step, P * QB + k + 1);
+                    if (lane == 0) {
+                        s_sig[k] = lp + 64 * rp;
+                        // LDS operations of a wave complete in order: a reader that sees
+                        // the step count also sees this step's multipliers
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        lds_store_relaxed(&s_step, P * QB + k + 1);
+                    }
                 }
             });
             // ---- publish: pivots, slot contents, final position -> slot
             if (lane < np) s.ipiv[j0 + lane] = (i64)(j0 + myp + 1);
-            T *pb = pbuf + b * RP * QB;
+            T *pb = pbuf + b * RP * QS;
 #pragma unroll
             for (int r = 0; r < PR; ++r) {
                 const int q = lane + 64 * r;
                 if (q < RP) {
 #pragma unroll
-                    for (int c = 0; c < QB; ++c) pb[q * QB + c] = pp[r][c];
+                    for (int c = 0; c < QB; ++c) pb[q * QS + c] = pp[r][c];
                     const int f = pos[r] < QBIG ? pos[r] : q;
                     s_src[b][f] = q;
                 }
@@ -375,13 +482,13 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
         auto write_u11 = [&](int Pp) {
             // U11 of panel Pp: U[k][c] = row of the pivot slot of step k, column c >= k
             const int j0 = Pp * QB, np = qmin(QB, mn - j0), b = Pp & 1;
-            const T *pb = pbuf + b * RP * QB;
+            const T *pb = pbuf + b * RP * QS;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int k = (lane >> 4) + 4 * i;
                 if (k < np && k <= c16) {
                     const int q = s_src[b][k];
-                    gstore(j0 + k, j0 + c16, pb[q * QB + c16]);
+                    gstore(j0 + k, j0 + c16, pb[q * QS + c16]);
                 }
             }
         };
@@ -408,7 +515,7 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
                     // row k: X(k, j) = [k == j] - sum_{k' < k} L11(k, k') X(k', j); colb holds -l
                     T x = (lane == k) ? T(1) : T(0);
 #pragma unroll
-                    for (int kk = 0; kk < k; ++kk) x = dfma(colb[kk * RP + q], X[kk], x);
+                    for (int kk = 0; kk < k; ++kk) x = dfma(colb[kk * (64 * PR) + q], X[kk], x);
                     X[k] = x;
                 }
             });
@@ -431,56 +538,144 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
         // ================================================================ update waves
         const int w = wv - 2;
         const int g = lane >> 4, li = lane & 15;
-        // Apply panel Pp to block column B.  mode 0: AB -> AB; mode 1: AB -> look-ahead
-        // buffer (next panel's positions); mode 2: look-ahead buffer -> next panel input,
-        // tiles rt0, rt0+QUW, ... only, U rows stored when store_u.
-        auto update_bc = [&](int Pp, int B, int mode, int rt0, int rstep, bool store_u) {
+        // Apply panel Pp to block column B, reading and writing AB (mode 0) or
+        // writing the rows the next panel needs to the look-ahead buffer (tola).
+        // Every load of the block column is issued before the first is used
+        // (one memory round trip per block column); loads and stores go
+        // through a buffer resource over the block column's 16 AB columns, an
+        // out-of-band element reading as 0 and its store being dropped.
+        auto update_glb = [&](int Pp, int B, bool tola) {
+            constexpr int MT = 4 * PR - 1;  // most 16-row tiles below the U rows
             const int j0p = Pp * QB, npp = qmin(QB, mn - j0p), b = Pp & 1;
-            const int c0 = B * QB;
+            const int c0 = B * QB, ncols = qmin(QB, n - c0);
             const int *src = s_src[b];
-            const T *pb = pbuf + b * RP * QB;
-            // (P A)[0:16] as the B operand: rows f = 4g+i, column c0+li
+            const T *pb = pbuf + b * RP * QS;
+            const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+                ab + (i64)c0 * ld, 0, (int)((i64)ncols * ld * (i64)sizeof(T)), kQRsrcWord3);
+            // element (r, c0+x) of AB sits (x*(ld-1) + kv + r - c0) elements into the
+            // block column; it is stored iff r < m, r - kl - c0 <= x <= r + kv - c0
+            // and c0 + x < n
+            const int cmax = n - 1 - c0;
+            auto offs = [&](int r, int c) -> int {
+                const int x = c - c0;
+                const bool ok = r < m && x >= r - kl - c0 && x <= qmin(r + kv - c0, cmax);
+                return ok ? (int)__umul24((unsigned)x, (unsigned)((ldi - 1) * (int)sizeof(T))) +
+                                (kv + r - c0) * (int)sizeof(T)
+                          : kQOob;
+            };
             T braw[4];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int f = 4 * g + i;
-                const int q = src[f];
-                T v = T(0);
-                if (f < npp) v = (mode == 2) ? lab[q * QB + li] : gload(j0p + q, c0 + li);
-                braw[i] = v;
+                braw[i] = qbload<T>(rs, f < npp ? offs(j0p + src[f], c0 + li) : kQOob);
+            }
+            const T *lv = linv + b * QB * QB + li * QB + 4 * g;
+            v4 u = {T(0), T(0), T(0), T(0)};
+            // tiles in chunks of CH (FP64 keeps fewer loads in flight to stay
+            // within the register budget); a chunk's stores never overwrite a
+            // row a later chunk reads (those come from the row itself or from
+            // the U rows' positions, written last)
+            constexpr int CH = sizeof(T) == 8 ? 6 : MT;
+            constexpr int NCH = (MT + CH - 1) / CH;
+#pragma unroll
+            for (int ch = 0; ch < NCH; ++ch) {
+                T acc[CH][4];
+                int qv[CH];
+#pragma unroll
+                for (int t = 0; t < CH; ++t) {
+                    const int rt = ch * CH + t + 1;
+                    if (rt < RT) {
+                        const int q = src[16 * rt + li];
+                        qv[t] = q;
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) acc[t][v] = qbload<T>(rs, offs(j0p + q, c0 + drow<T>(g, v)));
+                    }
+                }
+                if (ch == 0) {
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) u = Mf<T>::mma(lv[i], braw[i], u);
+                }
+#pragma unroll
+                for (int t = 0; t < CH; ++t) {
+                    const int rt = ch * CH + t + 1;
+                    if (rt < RT) {
+                        const int f = 16 * rt + li;
+                        v4 a = {acc[t][0], acc[t][1], acc[t][2], acc[t][3]};
+                        const T *lr = pb + qv[t] * QS;
+#pragma unroll
+                        for (int i = 0; i < 4; ++i) a = Mf<T>::mma(u[i], -lr[drow<T>(g, i)], a);
+                        if (f < R) {
+                            if (tola) {
+#pragma unroll
+                                for (int v = 0; v < 4; ++v) lab[(f - QB) * QS + drow<T>(g, v)] = a[v];
+                            } else {
+#pragma unroll
+                                for (int v = 0; v < 4; ++v)
+                                    qbstore<T>(rs, offs(j0p + f, c0 + drow<T>(g, v)), a[v]);
+                            }
+                        }
+                    }
+                }
+            }
+            // U rows, after every tile has read its rows (the rows displaced by
+            // this panel's pivots are read from the positions the U rows go to)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int f = drow<T>(g, v);
+                qbstore<T>(rs, f < npp ? offs(j0p + f, c0 + li) : kQOob, u[v]);
+            }
+            if (tola) {
+                // rows entering: look-ahead buffer positions R-16.. and the next
+                // panel's input positions R-16.. (original rows, untouched so far)
+                const int j0 = Pp * QB + QB;
+                T *pn = pan + ((Pp + 2) & 1) * RP * QS;
+                T e0[4], e1[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int e = lane + 64 * i, r = e >> 4, c = e & 15;
+                    e0[i] = qbload<T>(rs, offs(j0 + R - QB + r, c0 + c));
+                    e1[i] = qbload<T>(rs, offs(j0 + R + r, c0 + c));
+                }
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int e = lane + 64 * i, r = e >> 4, c = e & 15;
+                    lab[(R - QB + r) * QS + c] = e0[i];
+                    pn[(R - QB + r) * QS + c] = e1[i];
+                }
+            }
+        };
+        // Apply panel Pp to the look-ahead block column Pp+1 (LDS to LDS):
+        // tiles rt0, rt0 + QUW, ...; the U rows are stored when store_u.
+        auto update_la = [&](int Pp, int rt0, bool store_u) {
+            const int j0p = Pp * QB, npp = qmin(QB, mn - j0p), b = Pp & 1;
+            const int c0 = (Pp + 1) * QB;
+            const int *src = s_src[b];
+            const T *pb = pbuf + b * RP * QS;
+            T braw[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int f = 4 * g + i;
+                braw[i] = (f < npp) ? lab[src[f] * QS + li] : T(0);
             }
             const T *lv = linv + b * QB * QB + li * QB + 4 * g;
             v4 u = {T(0), T(0), T(0), T(0)};
 #pragma unroll
             for (int i = 0; i < 4; ++i) u = Mf<T>::mma(lv[i], braw[i], u);
-            T *dstp = (mode == 2) ? pan + ((Pp + 1) & 1) * RP * QB : lab;
-            for (int rt = rt0; rt < RT; rt += rstep) {
-                const int f = 16 * rt + li;  // final position of this lane's row
+            T *dstp = pan + ((Pp + 1) & 1) * RP * QS;
+            for (int rt = rt0; rt < RT; rt += QUW) {
+                const int f = 16 * rt + li;
                 const int q = src[f];
-                // acc[v]: row f, column c0 + drow(g,v) (the transposed product's result)
-                v4 acc;
+                v4 a;
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int c = drow<T>(g, v);
-                    acc[v] = (mode == 2) ? lab[q * QB + c] : gload(j0p + q, c0 + c);
-                }
-                // step i: k = drow(g, i) on both operands (u[i] is U12(drow(g,i), .))
-                const T *lr = pb + q * QB;
+                for (int v = 0; v < 4; ++v) a[v] = lab[q * QS + drow<T>(g, v)];
+                const T *lr = pb + q * QS;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) acc = Mf<T>::mma(u[i], -lr[drow<T>(g, i)], acc);
+                for (int i = 0; i < 4; ++i) a = Mf<T>::mma(u[i], -lr[drow<T>(g, i)], a);
                 if (f < R) {
-                    if (mode == 0) {
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) gstore(j0p + f, c0 + drow<T>(g, v), acc[v]);
-                    } else {
-#pragma unroll
-                        for (int v = 0; v < 4; ++v) dstp[(f - QB) * QB + drow<T>(g, v)] = acc[v];
-                    }
+                    for (int v = 0; v < 4; ++v) dstp[(f - QB) * QS + drow<T>(g, v)] = a[v];
                 }
             }
-            // u[v] = U12(drow(g,v), c0+li).  Stored after every tile has read its
-            // rows: the rows displaced by this panel's pivots are read from the
-            // positions the U rows go to.
             if (store_u) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
@@ -499,26 +694,13 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
                 const int Bend = la ? qmax(Bmax, P + 1) : Bmax;
                 for (int B = P + 1; B <= Bend; ++B) {
                     if (B % QUW != w) continue;
-                    const bool tola = la && B == P + 1;
-                    update_bc(Pp, B, tola ? 1 : 0, 1, 1, true);
-                    if (tola) {
-                        // rows entering: look-ahead buffer positions R-16.. and the
-                        // next panel's input positions R-16.. (untouched original rows)
-                        const int j0 = P * QB, c0 = B * QB;
-                        T *pn = pan + ((P + 1) & 1) * RP * QB;
-#pragma unroll
-                        for (int i = 0; i < 4; ++i) {
-                            const int e = lane + 64 * i, r = e >> 4, c = e & 15;
-                            lab[(R - QB + r) * QB + c] = gload(j0 + R - QB + r, c0 + c);
-                            pn[(R - QB + r) * QB + c] = gload(j0 + R + r, c0 + c);
-                        }
-                    }
+                    update_glb(Pp, B, la && B == P + 1);
                 }
             }
             if (P == NP) break;
             lds_barrier();  // B1
             // ---- phase 2: panel P onto the look-ahead block column P+1
-            if ((P + 1) * QB < n) update_bc(P, P + 1, 2, 1 + w, QUW, w == 0);
+            if ((P + 1) * QB < n) update_la(P, 1 + w, w == 0);
             lds_barrier();  // B2
         }
     }
@@ -526,7 +708,7 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
 
 template <typename T, int PR>
 int mfma_launch_one(const QArgs<T> &a, hipStream_t st) {
-    const size_t lds = sizeof(T) * ((size_t)6 * a.RP * QB + 2 * QB * QB);
+    const size_t lds = sizeof(T) * ((size_t)5 * a.RP * QStride<T>::value + (size_t)QB * 64 * PR + 2 * QB * QB);
     auto kern = gbtrf_mfma_kernel<T, PR>;
     if (lds > 64 * 1024)
         BH_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -552,8 +734,9 @@ __global__ void gbtrf_zero_fill(T *ab, i64 ld, int n, int mn, int kl, int ku) {
 // LDS bytes the kernel needs for a band, or 0 when it does not apply.
 size_t mfma_lds_bytes(i64 kl, i64 elem) {
     if (kl < 1 || kl > 240) return 0;
-    const i64 R = kl + QB, RP = 16 * ((R + 15) / 16);
-    return (size_t)elem * ((size_t)6 * RP * QB + 2 * QB * QB);
+    const i64 R = kl + QB, RP = 16 * ((R + 15) / 16), PR = (R + 63) / 64;
+    const i64 QSe = elem == 4 ? 20 : 18;
+    return (size_t)elem * ((size_t)5 * RP * QSe + (size_t)QB * 64 * PR + 2 * QB * QB);
 }
 
 // The blocked MFMA band LU, or -1 when the band does not fit it (kl + 16 rows
@@ -561,7 +744,9 @@ size_t mfma_lds_bytes(i64 kl, i64 elem) {
 template <typename T>
 int mfma_gbtrf_launch(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, i64 *dinfo, hipStream_t st) {
     const size_t lds = mfma_lds_bytes(kl, sizeof(T));
-    if (lds == 0 || lds > 160 * 1024 || m >= (1ll << 29) || n >= (1ll << 29) || kl + ku >= (1 << 20)) return -1;
+    if (lds == 0 || lds > 160 * 1024 || m >= (1ll << 29) || n >= (1ll << 29) || kl + ku >= (1 << 20) ||
+        (i64)QB * ldab * (i64)sizeof(T) >= (1ll << 31) || ldab * (i64)sizeof(T) >= (1ll << 24))
+        return -1;
     const int R = (int)kl + QB, RT = (R + 15) / 16, RP = 16 * RT;
     QArgs<T> a{ab, ipiv, dinfo, (int)m, (int)n, (int)kl, (int)ku, ldab, R, RT, RP};
     const int mn = (int)imin(m, n);
