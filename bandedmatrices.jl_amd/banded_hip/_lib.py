@@ -48,6 +48,7 @@ for _p, _t in (("d", dbl), ("s", flt)):
     SIGNATURES[f"bh_{_p}gbqrf"] = (C.c_int, [i64, i64, i64, i64, vp, i64, vp])
     SIGNATURES[f"bh_{_p}gbqrm"] = (C.c_int, [ch, i64, i64, i64, i64, vp, i64, vp, i64, vp, i64])
     SIGNATURES[f"bh_{_p}gbqrs"] = (C.c_int, [i64, i64, i64, i64, vp, i64, vp, i64, vp, i64])
+    SIGNATURES[f"bh_{_p}gbqrmr"] = (C.c_int, [ch, i64, i64, i64, i64, i64, vp, i64, vp, vp, i64])
     SIGNATURES[f"bh_{_p}tbsv"] = (C.c_int, [ch, ch, ch, i64, i64, vp, i64, vp, i64])
     SIGNATURES[f"bh_{_p}tbmv"] = (C.c_int, [ch, ch, ch, i64, i64, vp, i64, vp, i64])
     SIGNATURES[f"bh_{_p}tbsv_multi"] = (C.c_int, [ch, ch, ch, i64, i64, vp, i64, i64, vp, i64])

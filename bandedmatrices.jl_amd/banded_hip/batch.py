@@ -18,10 +18,13 @@ from .device import DeviceArray
 
 
 class BatchGraph:
-    """A captured batched solve bound to its plan's buffers; launch() replays it."""
+    """A captured batched solve bound to its plan's buffers; launch() replays it.
+    It keeps the BatchedGBSV (plan and buffers) alive; while it exists the
+    plan's schedule cannot change (schedule/autotune raise)."""
 
-    def __init__(self, handle):
+    def __init__(self, handle, owner):
         self._h = handle
+        self._owner = owner
 
     def launch(self):
         _lib.call("bh_batch_graph_launch", self._h)
@@ -38,6 +41,7 @@ class BatchedGBSV:
     """A batch of square band systems with bands (kl_s, ku_s), kl_s <= 8, kl_s + ku_s <= 63."""
 
     def __init__(self, n, kl, ku, dtype=np.float64):
+        self._graphs = []
         self.n = np.asarray(n, np.int64)
         self.kl = np.asarray(kl, np.int64)
         self.ku = np.asarray(ku, np.int64)
@@ -104,7 +108,9 @@ class BatchedGBSV:
         g = C.c_void_p()
         _lib.call(f"bh_{_prefix(self.dtype)}gbsv_batched_graph", self._plan, self.AB.ptr, self.B.ptr, self.ipiv.ptr,
                   self.info.ptr, C.byref(g))
-        return BatchGraph(g)
+        bg = BatchGraph(g, self)
+        self._graphs.append(bg)
+        return bg
 
     def results(self):
         """(x, factors, ipiv, info) per system as host arrays."""
@@ -118,6 +124,9 @@ class BatchedGBSV:
         return out
 
     def close(self):
+        for g in getattr(self, "_graphs", []):
+            g.close()
+        self._graphs = []
         if getattr(self, "_plan", None):
             _lib.load().bh_batch_plan_destroy(self._plan)
             self._plan = None

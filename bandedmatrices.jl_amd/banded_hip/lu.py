@@ -147,7 +147,7 @@ def ldiv_(F, B: DeviceArray) -> DeviceArray:
     nrhs = 1 if len(B.shape) == 1 else B.shape[1]
     p = _prefix(B.dtype)
     _lib.call(f"bh_{p}gbtrs", trans.encode(), n, F.kl, F.ku, nrhs, F.factors.data.ptr, F.factors.data.ld,
-              F.ipiv_dev.ptr, B.ptr, max(1, B.shape[0]))
+              F.ipiv_dev.ptr, B.ptr, B.ld)
     return B
 
 

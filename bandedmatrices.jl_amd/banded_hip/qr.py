@@ -2,6 +2,7 @@
 
   qr / qr!            src/banded/bandedqr.jl:1-35 (padded copy, _banded_qr!)
   Q' * B, Q * B       src/banded/bandedqr.jl:38-94,150-172 (banded_qr_lmul!)
+  B * Q, B * Q'       src/banded/bandedqr.jl:96-148,210-214 (banded_qr_rmul!)
   ldiv!(F::QR, B)     src/banded/bandedqr.jl:175-206 (Q'B then tbsv of R)
 The factors stay in HBM: R in the upper l+u bands of the padded copy, the
 reflectors in the lower l band, tau as a device vector.
@@ -61,6 +62,24 @@ class QRPackedQ:
     def __matmul__(self, B: DeviceArray) -> DeviceArray:
         return self.lmul_(B.copy())
 
+    def __rmatmul__(self, A: DeviceArray) -> DeviceArray:
+        return qr_rmul_(A.copy(), self)
+
+
+def qr_rmul_(A: DeviceArray, Q: QRPackedQ) -> DeviceArray:
+    """rmul!(A, Q) / rmul!(A, Q') in place: A (mA x m) <- A Q or A Q'
+    (banded_qr_rmul!, src/banded/bandedqr.jl:96-148,210-214)."""
+    F = Q.parent
+    m, n = F.shape
+    if len(A.shape) != 2 or A.shape[1] != m:
+        raise DimensionMismatch(f"matrix A has dimensions {A.shape} but matrix Q has dimensions ({m}, {n})")
+    if A.dtype != F.factors.dtype:
+        raise TypeError("mixed element types")
+    l, u = F.factors.bandwidths()
+    _lib.call(f"bh_{_prefix(A.dtype)}gbqrmr", b"T" if Q.adjoint else b"N", A.shape[0], m, n, l, u,
+              F.factors.data.ptr, F.factors.data.ld, F.tau.ptr, A.ptr, A.ld)
+    return A
+
 
 def qr_(A: BandedMatrix) -> BandedQR:
     """qr!(A): A already carries the l extra super-diagonals (bands (l, l+u))."""
@@ -87,7 +106,7 @@ def qr_lmul_(F: BandedQR, B: DeviceArray, trans: str) -> DeviceArray:
     nrhs = 1 if len(B.shape) == 1 else B.shape[1]
     l, u = F.factors.bandwidths()
     _lib.call(f"bh_{_prefix(B.dtype)}gbqrm", trans.encode(), m, n, l, u, F.factors.data.ptr, F.factors.data.ld,
-              F.tau.ptr, nrhs, B.ptr, max(1, B.shape[0]))
+              F.tau.ptr, nrhs, B.ptr, B.ld)
     return B
 
 
@@ -103,7 +122,7 @@ def qr_ldiv_(F: BandedQR, B: DeviceArray) -> DeviceArray:
     nrhs = 1 if len(B.shape) == 1 else B.shape[1]
     l, u = F.factors.bandwidths()
     _lib.call(f"bh_{_prefix(B.dtype)}gbqrs", m, n, l, u, F.factors.data.ptr, F.factors.data.ld, F.tau.ptr, nrhs,
-              B.ptr, max(1, B.shape[0]))
+              B.ptr, B.ld)
     return B
 
 

@@ -20,6 +20,11 @@ using i64 = int64_t;
 // Library-wide state: one current stream per host thread is enough for the
 // LAPACK-like calling convention (the caller serialises calls on a handle).
 hipStream_t stream();
+// Library-owned device memory: 32 control words (info, flags) and a scratch
+// buffer of at least `bytes` (grown, never shrunk; shared by every call on
+// the library stream).
+int ctrl_words(i64 **p);
+int scratch_buffer(size_t bytes, void **p);
 void set_error(const char *fmt, ...);
 int hip_fail(hipError_t e, const char *what, const char *file, int line);
 

@@ -98,7 +98,12 @@ int outside_check(i64 m, i64 n, const T *X, i64 ldx, i64 xl, i64 xu, i64 yl, i64
     if (!(xl > yl || xu > yu) || xl + xu + 1 <= 0 || n == 0 || m == 0) return BH_OK;
     hipStream_t st = stream();
     int *flag = nullptr;
-    BH_HIP(hipMallocAsync((void **)&flag, sizeof(int), st));
+    {
+        i64 *cw = nullptr;
+        const int cs = ctrl_words(&cw);
+        if (cs) return cs;
+        flag = reinterpret_cast<int *>(cw + 4);
+    }
     int hflag = 0;
     hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), st);
     if (e == hipSuccess) {
@@ -108,9 +113,7 @@ int outside_check(i64 m, i64 n, const T *X, i64 ldx, i64 xl, i64 xu, i64 yl, i64
     }
     if (e == hipSuccess) e = hipMemcpyAsync(&hflag, flag, sizeof(int), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    const hipError_t ef = hipFreeAsync(flag, st);
     if (e != hipSuccess) return hip_fail(e, "band check", __FILE__, __LINE__);
-    BH_HIP(ef);
     if (hflag) {
         set_error("BandError: X has a nonzero outside the bands (%lld, %lld) of the destination", (long long)yl,
                   (long long)yu);

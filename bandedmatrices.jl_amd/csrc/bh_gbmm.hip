@@ -143,14 +143,16 @@ template <typename T>
 static int band_flags(i64 rows, i64 cols, i64 l, i64 u, const T *A, i64 lda, int *hflags) {
     const i64 h = l + u + 1;
     int *d = nullptr;
-    BH_HIP(hipMallocAsync((void **)&d, h * sizeof(int), stream()));
+    {
+        const int cs = scratch_buffer((size_t)h * sizeof(int), (void **)&d);
+        if (cs) return cs;
+    }
     BH_HIP(hipMemsetAsync(d, 0, h * sizeof(int), stream()));
     if (h * cols > 0)
         hipLaunchKernelGGL(band_nonzero<T>, dim3((unsigned)cdiv(h * cols, 256)), dim3(256), 0, stream(), rows, cols,
                            l, u, A, lda, d);
     BH_LAUNCH_CHECK();
     BH_HIP(hipMemcpyAsync(hflags, d, h * sizeof(int), hipMemcpyDeviceToHost, stream()));
-    BH_HIP(hipFreeAsync(d, stream()));
     BH_HIP(hipStreamSynchronize(stream()));
     return BH_OK;
 }

@@ -298,6 +298,49 @@ __global__ void gbqr_apply_serial(bool transpose, int m, int n, int l, int u, co
     }
 }
 
+// A Q / A Q' (banded_qr_rmul!): one thread per row of A; the reflectors are
+// applied in sequence (ascending for Q, descending for Q'), each to columns
+// k .. min(k+l, mQ-1) of the thread's row, in the restated order of sums.
+template <typename T>
+__global__ void __launch_bounds__(256) gbqr_rmul_rows(bool transpose, int mA, int mQ, int kmax, int l, int u,
+                                                      const T *__restrict__ D, i64 ld, const T *__restrict__ tau,
+                                                      T *A, i64 lda) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= mA) return;
+    T *a = A + i;
+    for (int t = 0; t < kmax; ++t) {
+        const int k = transpose ? kmax - 1 - t : t;
+        const int jhi = (k + l < mQ - 1) ? k + l : mQ - 1;
+        const T *x = D + (i64)k * ld + u - k;  // x[j] = v_j for j in k+1 .. jhi
+        T v = a[(i64)k * lda];
+        for (int j = k + 1; j <= jhi; ++j) v = v + a[(i64)j * lda] * x[j];
+        v = v * tau[k];
+        a[(i64)k * lda] = a[(i64)k * lda] - v;
+        for (int j = k + 1; j <= jhi; ++j) a[(i64)j * lda] = a[(i64)j * lda] - v * x[j];
+    }
+}
+
+template <typename T>
+int gbqrmr_impl(char trans, i64 mA, i64 mQ, i64 nQ, i64 l, i64 u, const T *D, i64 ldd, const T *tau, T *A,
+                i64 lda) {
+    BH_ARG(trans == 'N' || trans == 'n' || trans == 'T' || trans == 't' || trans == 'C' || trans == 'c', 1);
+    BH_ARG(mA >= 0, 2);
+    BH_ARG(mQ >= 0, 3);
+    BH_ARG(nQ >= 0, 4);
+    BH_ARG(l >= 0, 5);
+    BH_ARG(u >= 0, 6);
+    BH_ARG(ldd >= l + u + 1, 8);
+    BH_ARG(lda >= imax(1, mA), 11);
+    BH_ARG(mA < (1ll << 31) && mQ < (1ll << 31), 2);
+    const i64 kmax = imin(mQ, nQ);
+    if (mA == 0 || kmax == 0) return BH_OK;
+    const bool tr = !(trans == 'N' || trans == 'n');
+    hipLaunchKernelGGL(gbqr_rmul_rows<T>, dim3((unsigned)cdiv(mA, 256)), dim3(256), 0, stream(), tr, (int)mA,
+                       (int)mQ, (int)kmax, (int)l, (int)u, D, ldd, tau, A, lda);
+    BH_LAUNCH_CHECK();
+    return BH_OK;
+}
+
 template <typename T>
 int gbqrf_impl(i64 m, i64 n, i64 l, i64 u, T *D, i64 ldd, T *tau) {
     BH_ARG(m >= 0, 1);
@@ -350,6 +393,13 @@ int gbqrm_impl(char trans, i64 m, i64 n, i64 l, i64 u, const T *D, i64 ldd, cons
     return BH_OK;
 }
 
+static int qr_tbsv_upper(i64 n, i64 k, const double *R, i64 ldr, i64 nrhs, double *B, i64 ldb) {
+    return bh_dtbsv_multi('U', 'N', 'N', n, k, R, ldr, nrhs, B, ldb);
+}
+static int qr_tbsv_upper(i64 n, i64 k, const float *R, i64 ldr, i64 nrhs, float *B, i64 ldb) {
+    return bh_stbsv_multi('U', 'N', 'N', n, k, R, ldr, nrhs, B, ldb);
+}
+
 template <typename T>
 int gbqrs_impl(i64 m, i64 n, i64 l, i64 u, const T *D, i64 ldd, const T *tau, i64 nrhs, T *B, i64 ldb) {
     BH_ARG(m >= n, 1);
@@ -362,12 +412,11 @@ int gbqrs_impl(i64 m, i64 n, i64 l, i64 u, const T *D, i64 ldd, const T *tau, i6
     if (n == 0 || nrhs == 0) return BH_OK;
     int st = gbqrm_impl<T>('T', m, n, l, u, D, ldd, tau, nrhs, B, ldb);
     if (st != BH_OK) return st;
-    // R \ (Q'B)[1:n, :]: back substitution with u super-diagonals, no L part
+    // R \ (Q'B)[1:n, :]: back substitution with u super-diagonals, no L part;
+    // bands beyond the lane-ring solve's reach take the triangular band solve
+    // (the tbsv of the reference's ldiv!, src/banded/tribanded.jl:75-83)
     st = wave_gbtrs_launch<T>(true, n, 0, u, nrhs, D, ldd, nullptr, B, ldb, stream());
-    if (st == -1) {
-        set_error("bh_gbqrs: upper bandwidth %lld is beyond the solve kernel's 319", (long long)u);
-        return -4;
-    }
+    if (st == -1) st = qr_tbsv_upper(n, u, D, ldd, nrhs, B, ldb);
     return st;
 }
 
@@ -391,6 +440,14 @@ int bh_dgbqrm(char trans, bh_int m, bh_int n, bh_int l, bh_int u, const double *
 int bh_sgbqrm(char trans, bh_int m, bh_int n, bh_int l, bh_int u, const float *D, bh_int ldd, const float *tau,
               bh_int nrhs, float *B, bh_int ldb) {
     return gbqrm_impl<float>(trans, m, n, l, u, D, ldd, tau, nrhs, B, ldb);
+}
+int bh_dgbqrmr(char trans, bh_int mA, bh_int mQ, bh_int nQ, bh_int l, bh_int u, const double *D, bh_int ldd,
+               const double *tau, double *A, bh_int lda) {
+    return gbqrmr_impl<double>(trans, mA, mQ, nQ, l, u, D, ldd, tau, A, lda);
+}
+int bh_sgbqrmr(char trans, bh_int mA, bh_int mQ, bh_int nQ, bh_int l, bh_int u, const float *D, bh_int ldd,
+               const float *tau, float *A, bh_int lda) {
+    return gbqrmr_impl<float>(trans, mA, mQ, nQ, l, u, D, ldd, tau, A, lda);
 }
 int bh_dgbqrs(bh_int m, bh_int n, bh_int l, bh_int u, const double *D, bh_int ldd, const double *tau, bh_int nrhs,
               double *B, bh_int ldb) {

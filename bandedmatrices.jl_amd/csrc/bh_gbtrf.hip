@@ -170,7 +170,10 @@ static int gbtrf_impl(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, 
     *info_host = 0;
     if (m == 0 || n == 0) return BH_OK;
     i64 *dinfo = nullptr;
-    BH_HIP(hipMallocAsync((void **)&dinfo, 2 * sizeof(i64), stream()));
+    {
+        const int cs = ctrl_words(&dinfo);
+        if (cs) return cs;
+    }
     BH_HIP(hipMemsetAsync(dinfo, 0, 2 * sizeof(i64), stream()));
     // register-resident kernel when the window fits one of its shapes,
     // otherwise the barrier-stepped kernel working in global memory
@@ -194,7 +197,6 @@ static int gbtrf_impl(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, 
     }
     i64 hinfo[2] = {0, 0};
     BH_HIP(hipMemcpyAsync(hinfo, dinfo, 2 * sizeof(i64), hipMemcpyDeviceToHost, stream()));
-    BH_HIP(hipFreeAsync(dinfo, stream()));
     BH_HIP(hipStreamSynchronize(stream()));
     if (hinfo[1] != 0) {
         set_error("gbtrf: an internal wait of the blocked kernel gave up; the factors are not valid");

@@ -315,12 +315,22 @@ struct bh_batch_plan {
     BatchDesc *desc = nullptr;  // device, schedule order, padded to whole rounds
     i64 desc_len = 0, desc_cap = 0;
     int waves_per_block = 1, blocks = 0, cus = 256, order_kind = 0;
+    // graphs captured from this plan: each baked in desc, desc_len and the launch
+    // geometry, so the schedule may not change (nor the plan go) while any exists
+    int live_graphs = 0;
 };
 
 struct bh_batch_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    bh_batch_plan *plan = nullptr;
 };
+
+static int plan_busy(const bh_batch_plan *p, const char *what) {
+    if (p->live_graphs == 0) return BH_OK;
+    bh::set_error("%s: the plan has %d captured graph(s); destroy them first", what, p->live_graphs);
+    return BH_ERR_BUSY;
+}
 
 namespace {
 
@@ -484,6 +494,8 @@ int batch_graph_create(bh_batch_plan *p, T *AB, T *B, i64 *ipiv, i64 *info, bh_b
         delete g;
         return st;
     }
+    g->plan = p;
+    ++p->live_graphs;
     *out = g;
     return BH_OK;
 }
@@ -550,6 +562,10 @@ int bh_batch_plan_schedule(bh_batch_plan *p, int order_kind, int waves_per_block
     BH_ARG(order_kind >= 0 && order_kind <= 2, 2);
     BH_ARG(waves_per_block >= 1 && waves_per_block <= 4, 3);
     BH_ARG(blocks >= 1, 4);
+    {
+        const int b = plan_busy(p, "bh_batch_plan_schedule");
+        if (b) return b;
+    }
     p->waves_per_block = waves_per_block;
     p->blocks = blocks;
     return plan_set_order(p, order_kind);
@@ -565,6 +581,10 @@ int bh_batch_plan_get_schedule(const bh_batch_plan *p, int *order_kind, int *wav
 
 int bh_batch_plan_destroy(bh_batch_plan *p) {
     if (!p) return BH_OK;
+    {
+        const int b = plan_busy(p, "bh_batch_plan_destroy");
+        if (b) return b;
+    }
     (void)hipFree(p->desc);
     delete p;
     return BH_OK;
@@ -580,9 +600,11 @@ int bh_sgbsv_batched(bh_batch_plan *p, float *AB, float *B, bh_int *ipiv, bh_int
 }
 
 int bh_dgbsv_batched_autotune(bh_batch_plan *p, const double *AB, const double *B, double *best_ms) {
+    if (p && p->live_graphs) return plan_busy(p, "bh_dgbsv_batched_autotune");
     return batch_autotune<double>(p, AB, B, best_ms);
 }
 int bh_sgbsv_batched_autotune(bh_batch_plan *p, const float *AB, const float *B, double *best_ms) {
+    if (p && p->live_graphs) return plan_busy(p, "bh_sgbsv_batched_autotune");
     return batch_autotune<float>(p, AB, B, best_ms);
 }
 
@@ -602,6 +624,7 @@ int bh_batch_graph_destroy(bh_batch_graph *g) {
     if (!g) return BH_OK;
     if (g->exec) hipGraphExecDestroy(g->exec);
     if (g->graph) hipGraphDestroy(g->graph);
+    if (g->plan) --g->plan->live_graphs;
     delete g;
     return BH_OK;
 }

@@ -8,8 +8,11 @@
 //      right-hand sides); stop when every column satisfies LAPACK dsgesv's
 //      rule ||r||_inf <= sqrt(n) * eps64 * ||A||_inf * ||x||_inf; else
 //      d = FP32 solve of r, x += d (FP64 accumulation);
-//   4. after 30 iterations without convergence (or an FP32 overflow or a zero
-//      FP32 pivot) fall back to FP64 LU + solve, reported as iters < 0.
+//   4. after 30 iterations without convergence, or as soon as refinement
+//      stagnates (the largest ratio ||r_c|| / ||x_c|| over the columns fails to
+//      halve from one iteration to the next, twice running), or on an FP32
+//      overflow or a zero FP32 pivot, fall back to FP64 LU + solve, reported
+//      as iters < 0.
 // Residuals, norms and the solution are FP64 throughout.
 #include <math.h>
 
@@ -181,6 +184,8 @@ int bh_dsgbsv(bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const double *A, bh_i
         hipLaunchKernelGGL(s2d_add_kernel, grid1(n * nrhs), dim3(256), 0, s, n, nrhs, w.r32, n, X, ldx, 1);
         BH_LAUNCH_CHECK();
         std::vector<double> nh(1 + 2 * nrhs);
+        double prev_ratio = HUGE_VAL;
+        int slow = 0;
         for (iter = 0; iter <= itmax; ++iter) {
             // r = B - A x (FP64)
             BH_HIP(hipMemcpy2DAsync(w.r, n * sizeof(double), B, ldb * sizeof(double), n * sizeof(double), nrhs,
@@ -196,15 +201,22 @@ int bh_dsgbsv(bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const double *A, bh_i
             BH_HIP(hipMemcpyAsync(nh.data(), w.norms, (1 + 2 * nrhs) * sizeof(double), hipMemcpyDeviceToHost, s));
             BH_HIP(hipStreamSynchronize(s));
             bool ok = true;
+            double ratio = 0.0;
             for (i64 c = 0; c < nrhs; ++c) {
                 const double xn = nh[1 + c], rn = nh[1 + nrhs + c];
                 if (!(rn <= xn * cte)) ok = false;
+                const double q = rn / (xn > 0.0 ? xn : 1.0);
+                ratio = (q > ratio || q != q) ? q : ratio;
             }
             if (ok) {
                 converged = true;
                 break;
             }
             if (iter == itmax) break;
+            // stagnation: the residual ratio must at least halve per correction
+            slow = (iter >= 1 && !(ratio <= 0.5 * prev_ratio)) ? slow + 1 : 0;
+            if (slow >= 2) break;
+            prev_ratio = ratio;
             // d = A32 \ r ; x += d
             hipLaunchKernelGGL(d2s_kernel, grid1(n * nrhs), dim3(256), 0, s, n, nrhs, w.r, n, w.r32, n);
             BH_LAUNCH_CHECK();
@@ -218,11 +230,18 @@ int bh_dsgbsv(bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const double *A, bh_i
         *iters_host = iter;
         return BH_OK;
     }
-    // FP64 fallback: LU + solve (allocates the FP64 factors)
+    // FP64 fallback: LU + solve, the FP64 factors in the library's scratch
+    // buffer (grown on the first fallback of a size, reused afterwards)
     double *ab64 = nullptr;
     i64 *ipiv64 = nullptr;
-    BH_HIP(hipMallocAsync((void **)&ab64, (size_t)ldab * n * sizeof(double), s));
-    BH_HIP(hipMallocAsync((void **)&ipiv64, (size_t)n * sizeof(i64), s));
+    {
+        const size_t fb = align256((size_t)ldab * n * sizeof(double));
+        void *sp = nullptr;
+        st = scratch_buffer(fb + (size_t)n * sizeof(i64), &sp);
+        if (st) return st;
+        ab64 = (double *)sp;
+        ipiv64 = (i64 *)((char *)sp + fb);
+    }
     st = bh_dgbpad(n, n, kl, ku, A, lda, ab64, ldab);
     i64 info64 = 0;
     if (!st) st = bh_dgbtrf(n, n, kl, ku, ab64, ldab, ipiv64, &info64);
@@ -231,8 +250,6 @@ int bh_dsgbsv(bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const double *A, bh_i
                                 hipMemcpyDeviceToDevice, s));
         st = bh_dgbtrs('N', n, kl, ku, nrhs, ab64, ldab, ipiv64, X, ldx);
     }
-    BH_HIP(hipFreeAsync(ab64, s));
-    BH_HIP(hipFreeAsync(ipiv64, s));
     BH_HIP(hipStreamSynchronize(s));
     *info_host = info64;
     *iters_host = -(iter + 1);

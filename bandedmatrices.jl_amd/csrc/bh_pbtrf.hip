@@ -188,7 +188,11 @@ int pbtrf_impl(char uplo, i64 n, i64 kd, T *ab, i64 ldab, i64 *info_host) {
     const size_t lds = (size_t)(fixed + B * H * (i64)sizeof(T));
     hipStream_t st = stream();
     i64 *dinfo = nullptr;
-    BH_HIP(hipMallocAsync((void **)&dinfo, sizeof(i64), st));
+    {
+        const int cs = ctrl_words(&dinfo);
+        if (cs) return cs;
+        dinfo += 2;
+    }
     auto kern = up ? pbtrf_lds_kernel<T, true, LDL> : pbtrf_lds_kernel<T, false, LDL>;
     hipError_t e = hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e == hipSuccess) {
@@ -197,9 +201,7 @@ int pbtrf_impl(char uplo, i64 n, i64 kd, T *ab, i64 ldab, i64 *info_host) {
     }
     if (e == hipSuccess) e = hipMemcpyAsync(info_host, dinfo, sizeof(i64), hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    const hipError_t ef = hipFreeAsync(dinfo, st);
     if (e != hipSuccess) return hip_fail(e, "pbtrf", __FILE__, __LINE__);
-    BH_HIP(ef);
     return BH_OK;
 }
 

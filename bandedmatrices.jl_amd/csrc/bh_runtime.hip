@@ -10,6 +10,17 @@ static thread_local char g_err[512] = "";
 static thread_local hipStream_t g_user_stream = nullptr;
 static hipStream_t g_own_stream = nullptr;
 static int g_device = -1;
+// Library-owned device memory (allocated by bh_init, grown only when a call
+// needs more than any earlier one): the entry points take their info words,
+// flags and temporaries from here instead of allocating per call.  Calls are
+// stream-ordered on the library stream; calls that overlap on different
+// streams must not share the library (LAPACK-like reentrancy: one handle per
+// task).
+static i64 *g_ctrl = nullptr;
+static void *g_scratch = nullptr;
+static size_t g_scratch_bytes = 0;
+constexpr size_t kCtrlBytes = 256;
+constexpr size_t kScratchInitial = 8u << 20;
 
 void set_error(const char *fmt, ...) {
     va_list ap;
@@ -31,6 +42,30 @@ static int ensure_init() {
 hipStream_t stream() {
     if (g_user_stream) return g_user_stream;
     return g_own_stream;
+}
+
+int ctrl_words(i64 **p) {
+    int st = ensure_init();
+    if (st) return st;
+    *p = g_ctrl;
+    return BH_OK;
+}
+
+int scratch_buffer(size_t bytes, void **p) {
+    int st = ensure_init();
+    if (st) return st;
+    if (bytes > g_scratch_bytes) {
+        // grow: earlier work on the stream may still read the old buffer
+        BH_HIP(hipStreamSynchronize(stream()));
+        if (g_scratch) BH_HIP(hipFree(g_scratch));
+        g_scratch = nullptr;
+        g_scratch_bytes = 0;
+        const size_t nb = bytes + bytes / 2;
+        BH_HIP(hipMalloc(&g_scratch, nb));
+        g_scratch_bytes = nb;
+    }
+    *p = g_scratch;
+    return BH_OK;
 }
 
 }  // namespace bh
@@ -60,6 +95,13 @@ int bh_init(int device) {
     if (g_own_stream == nullptr || g_device != device) {
         // Default priority only: the library never changes stream priorities.
         BH_HIP(hipStreamCreateWithFlags(&g_own_stream, hipStreamNonBlocking));
+        // library-owned control words and scratch on this device
+        g_ctrl = nullptr;
+        g_scratch = nullptr;
+        g_scratch_bytes = 0;
+        BH_HIP(hipMalloc((void **)&g_ctrl, kCtrlBytes));
+        BH_HIP(hipMalloc(&g_scratch, kScratchInitial));
+        g_scratch_bytes = kScratchInitial;
     }
     g_device = device;
     return BH_OK;

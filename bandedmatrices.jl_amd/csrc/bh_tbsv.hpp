@@ -265,7 +265,10 @@ __global__ void __launch_bounds__(256) tb_scatter(int n, const T *__restrict__ y
 template <typename T, int MODE, bool UNIT>
 int tbmv_launch(int n, int k, const T *a, i64 lda, T *x, int incx, hipStream_t st) {
     T *y = nullptr;
-    BH_HIP(hipMallocAsync((void **)&y, (size_t)n * sizeof(T), st));
+    {
+        const int cs = scratch_buffer((size_t)n * sizeof(T), (void **)&y);
+        if (cs) return cs;
+    }
     const int nb = (int)cdiv(n, 256);
     hipLaunchKernelGGL((tbmv_rows<T, MODE, UNIT>), dim3(nb), dim3(256), 0, st, n, k, a, lda, (const T *)x, incx, y);
     hipError_t e = hipGetLastError();
@@ -273,9 +276,7 @@ int tbmv_launch(int n, int k, const T *a, i64 lda, T *x, int incx, hipStream_t s
         hipLaunchKernelGGL((tb_scatter<T>), dim3(nb), dim3(256), 0, st, n, (const T *)y, x, incx);
         e = hipGetLastError();
     }
-    const hipError_t ef = hipFreeAsync(y, st);
     if (e != hipSuccess) return hip_fail(e, "kernel launch", __FILE__, __LINE__);
-    BH_HIP(ef);
     return BH_OK;
 }
 

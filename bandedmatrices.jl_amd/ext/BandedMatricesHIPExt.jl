@@ -6,13 +6,14 @@
 # Project.toml:11-17) when both BandedMatrices and AMDGPU are present.
 # Every method below replaces one reference method for the device layout:
 #   mul!(y, A, x, α, β)   materialize!(::MulAdd{BandedColumns,...})  src/generic/matmul.jl:61-64
-#   A' * x                 BandedRows gbmv                            src/generic/matmul.jl:66-92
+#   A' * x, transpose(A)*x materialize!(::MulAdd{BandedRows,...})      src/generic/matmul.jl:66-92
 #   mul!(Y, A, X, α, β)    dense multi-RHS                            src/generic/matmul.jl:243-256
 #   A * B  (banded)        gbmm!                                      src/banded/gbmm.jl:207-340
 #   lu / lu!               LAPACK.gbtrf!                              src/banded/BandedLU.jl:90-111
 #   ldiv!(F, B), F' \ B    LAPACK.gbtrs!                              src/banded/linalg.jl:24-47
 #   A \ b                  materialize!(::Ldiv{BandedColumns})        src/banded/linalg.jl:5-9,75
-#   qr / ldiv!(QR, B)      _banded_qr!, banded_qr_lmul!, tbsv!        src/banded/bandedqr.jl:1-214
+#   qr / ldiv!(QR, B)      _qr/_qr!, ldiv!(::QR), tbsv!               src/banded/bandedqr.jl:1-5,175-206
+#   Q*B, Q'*B, B*Q, B*Q'   banded_qr_lmul!, banded_qr_rmul!           src/banded/bandedqr.jl:38-148,150-214
 #   Tri(A) \ x, Tri(A) * x tbsv! / tbmv! on the triangle's band rows  src/tribanded.jl:19-26,47-54,75-96
 #   lmul!/rmul!, axpy!     _banded_lmul!/_rmul!, _banded_axpy!         src/generic/broadcast.jl:942-1020
 #   copyto!(B, A) banded   copyto_bandeddata!                          src/banded/BandedMatrix.jl:124-130
@@ -67,6 +68,7 @@ for (T, p) in ((Float64, :d), (Float32, :s))
     gbmv = Symbol(:bh_, p, :gbmv); gbmv_multi = Symbol(:bh_, p, :gbmv_multi); gbmm = Symbol(:bh_, p, :gbmm)
     gbtrf = Symbol(:bh_, p, :gbtrf); gbtrs = Symbol(:bh_, p, :gbtrs); gbpad = Symbol(:bh_, p, :gbpad)
     gbqrf = Symbol(:bh_, p, :gbqrf); gbqrs = Symbol(:bh_, p, :gbqrs)
+    gbqrm = Symbol(:bh_, p, :gbqrm); gbqrmr = Symbol(:bh_, p, :gbqrmr)
     tbsv = Symbol(:bh_, p, :tbsv); tbmv = Symbol(:bh_, p, :tbmv)
     gbscal = Symbol(:bh_, p, :gbscal); gbaxpy = Symbol(:bh_, p, :gbaxpy); gbcopy = Symbol(:bh_, p, :gbcopy)
     pbtrf = Symbol(:bh_, p, :pbtrf); pbtrs = Symbol(:bh_, p, :pbtrs); sbmv = Symbol(:bh_, p, :sbmv)
@@ -215,6 +217,60 @@ for (T, p) in ((Float64, :d), (Float32, :s))
             end
             C
         end
+        # qr!: padded copy (BandedMatrix{T}(A,(l,l+u)), bandedqr.jl:2) then the band
+        # Householder QR (bandedqr.jl:8-35); R and the reflectors stay in the padded
+        # band, τ on the device.
+        function _qr(A::HIPBanded{$T})
+            l, u = bandwidths(A); m, n = size(A); d = bandeddata(A)
+            R = ROCMatrix{$T}(undef, 2l + u + 1, n)
+            τ = ROCVector{$T}(undef, max(min(m, n), 1))
+            with_stream() do
+                chk(ccall(($(QuoteNode(gbpad)), libbh), Cint, (Int64, Int64, Int64, Int64, Ptr{$T}, Int64, Ptr{$T}, Int64),
+                          m, n, l, u, dptr(d), stride(d, 2), dptr(R), stride(R, 2)), $(QuoteNode(gbpad)))
+                chk(ccall(($(QuoteNode(gbqrf)), libbh), Cint, (Int64, Int64, Int64, Int64, Ptr{$T}, Int64, Ptr{$T}),
+                          m, n, l, l + u, dptr(R), stride(R, 2), dptr(τ)), $(QuoteNode(gbqrf)))
+            end
+            QR(_BandedMatrix(R, m, l, l + u), τ)
+        end
+        # ldiv!(QR, B): Q'B then the upper band solve of R (bandedqr.jl:175-206)
+        function _gbqrs!(F::QR{$T,<:HIPBanded{$T}}, B::ROCVecOrMat{$T})
+            R = F.factors; m, n = size(R); l, u = bandwidths(R); d = bandeddata(R)
+            n > m && error("Not implemented")  # bandedqr.jl:175-177
+            size(B, 1) == m || throw(DimensionMismatch("B has $(size(B, 1)) rows, but needs $m"))
+            ldb = B isa ROCVector ? length(B) : stride(B, 2)
+            with_stream() do
+                chk(ccall(($(QuoteNode(gbqrs)), libbh), Cint,
+                          (Int64, Int64, Int64, Int64, Ptr{$T}, Int64, Ptr{$T}, Int64, Ptr{$T}, Int64),
+                          m, n, l, u, dptr(d), stride(d, 2), dptr(F.τ), size(B, 2), dptr(B), ldb), $(QuoteNode(gbqrs)))
+            end
+            B
+        end
+        # lmul!(Q, B) / lmul!(Q', B) (banded_qr_lmul!, bandedqr.jl:38-94)
+        function _gbqrm!(trans::Char, Q::LinearAlgebra.QRPackedQ{$T,<:HIPBanded{$T}}, B::ROCVecOrMat{$T})
+            R = Q.factors; m, n = size(R); l, u = bandwidths(R); d = bandeddata(R)
+            size(B, 1) == m || throw(DimensionMismatch("B has $(size(B, 1)) rows, Q is $m x $m"))
+            ldb = B isa ROCVector ? length(B) : stride(B, 2)
+            with_stream() do
+                chk(ccall(($(QuoteNode(gbqrm)), libbh), Cint,
+                          (UInt8, Int64, Int64, Int64, Int64, Ptr{$T}, Int64, Ptr{$T}, Int64, Ptr{$T}, Int64),
+                          trans, m, n, l, u, dptr(d), stride(d, 2), dptr(Q.τ), size(B, 2), dptr(B), ldb),
+                    $(QuoteNode(gbqrm)))
+            end
+            B
+        end
+        # rmul!(A, Q) / rmul!(A, Q') (banded_qr_rmul!, bandedqr.jl:96-148)
+        function _gbqrmr!(trans::Char, A::ROCMatrix{$T}, Q::LinearAlgebra.QRPackedQ{$T,<:HIPBanded{$T}})
+            R = Q.factors; m, n = size(R); l, u = bandwidths(R); d = bandeddata(R)
+            size(A, 2) == m ||
+                throw(DimensionMismatch("matrix A has dimensions $(size(A)) but matrix Q has dimensions ($m, $n)"))
+            with_stream() do
+                chk(ccall(($(QuoteNode(gbqrmr)), libbh), Cint,
+                          (UInt8, Int64, Int64, Int64, Int64, Int64, Ptr{$T}, Int64, Ptr{$T}, Ptr{$T}, Int64),
+                          trans, size(A, 1), m, n, l, u, dptr(d), stride(d, 2), dptr(Q.τ), dptr(A), stride(A, 2)),
+                    $(QuoteNode(gbqrmr)))
+            end
+            A
+        end
         # lu!: padded copy (BandedMatrix{T}(A,(l,l+u)), BandedLU.jl:110) then gbtrf
         # (BandedLU.jl:98).  ipiv is returned on the host, as BandedLU stores it.
         function _lu(A::HIPBanded{$T})
@@ -228,7 +284,9 @@ for (T, p) in ((Float64, :d), (Float32, :s))
                           (Int64, Int64, Int64, Int64, Ptr{$T}, Int64, Ptr{Int64}, Ref{Int64}),
                           m, n, l, u, dptr(AB), stride(AB, 2), dptr(ipiv_d), info), $(QuoteNode(gbtrf)))
             end
-            F = BandedLU{$T}(_BandedMatrix(AB, m, l, l + u), Vector{BlasInt}(Array(ipiv_d)), BlasInt(info[]))
+            # LAPACK.gbtrf! raises on a zero pivot (chklapackerror), so lu does too
+            info[] > 0 && throw(LinearAlgebra.LAPACKException(BlasInt(info[])))
+            F = BandedLU{$T}(_BandedMatrix(AB, m, l, l + u), Vector{BlasInt}(Array(ipiv_d)), BlasInt(0))
             IPIV_DEV[F.ipiv] = ipiv_d      # device mirror for gbtrs
             F
         end
@@ -255,13 +313,33 @@ materialize!(M::MulAdd{HIPBC,HIPColumnMajor,HIPColumnMajor,<:Any,<:HIPBanded,<:R
 materialize!(M::MulAdd{HIPBC,HIPColumnMajor,HIPColumnMajor,<:Any,<:HIPBanded,<:ROCMatrix,<:ROCMatrix}) =
     _gbmv_multi!('N', M.α, M.A, M.B, M.β, M.C)
 materialize!(M::MulAdd{HIPBC,HIPBC,HIPBC}) = _gbmm!(M.α, M.A, M.B, M.β, M.C)
+# transpose(A) * x and A' * x (real): the row-major band layout of the wrapper
+# (BlasMatMulVecAdd{<:BandedRowMajor}, src/generic/matmul.jl:66-92)
+const HIPBR = BandedMatrices.BandedRows{HIPColumnMajor}
+materialize!(M::MulAdd{HIPBR,HIPColumnMajor,HIPColumnMajor,<:Any,<:Any,<:ROCVector,<:ROCVector}) =
+    _gbmv!('T', M.α, parent(M.A), M.B, M.β, M.C)
 LinearAlgebra.mul!(y::ROCVector{T}, A::Adjoint{T,<:HIPBanded{T}}, x::ROCVector{T}, α::Number, β::Number) where {T<:Union{Float32,Float64}} =
+    _gbmv!('T', T(α), parent(A), x, T(β), y)
+LinearAlgebra.mul!(y::ROCVector{T}, A::Transpose{T,<:HIPBanded{T}}, x::ROCVector{T}, α::Number, β::Number) where {T<:Union{Float32,Float64}} =
     _gbmv!('T', T(α), parent(A), x, T(β), y)
 ArrayLayouts._lu(::HIPBC, axes, A::HIPBanded; kwds...) = _lu(A)
 LinearAlgebra.ldiv!(F::BandedLU{T,<:HIPBanded{T}}, B::ROCVecOrMat{T}) where {T} = _gbtrs!('N', F, B)
 LinearAlgebra.ldiv!(F::Transpose{T,<:BandedLU{T,<:HIPBanded{T}}}, B::ROCVecOrMat{T}) where {T} = _gbtrs!('T', parent(F), B)
 LinearAlgebra.ldiv!(F::Adjoint{T,<:BandedLU{T,<:HIPBanded{T}}}, B::ROCVecOrMat{T}) where {T<:Real} = _gbtrs!('T', parent(F), B)
-materialize!(M::Ldiv{HIPBC}) = ldiv!(lu(M.A), M.B)
+function materialize!(M::Ldiv{HIPBC})
+    LinearAlgebra.checksquare(M.A)  # src/banded/linalg.jl:5-9
+    ldiv!(lu(M.A), M.B)
+end
+# qr (bandedqr.jl:1-5) and the QR solve / Q products (bandedqr.jl:150-214)
+ArrayLayouts._qr(::HIPBC, axes, A::HIPBanded; kwds...) = _qr(A)
+ArrayLayouts._qr!(::HIPBC, axes, A::HIPBanded; kwds...) = _qr(A)
+LinearAlgebra.ldiv!(F::QR{T,<:HIPBanded{T}}, B::ROCVecOrMat{T}) where {T} = _gbqrs!(F, B)
+LinearAlgebra.lmul!(Q::LinearAlgebra.QRPackedQ{T,<:HIPBanded{T}}, B::ROCVecOrMat{T}) where {T} = _gbqrm!('N', Q, B)
+LinearAlgebra.lmul!(Q::LinearAlgebra.AdjointQ{T,<:LinearAlgebra.QRPackedQ{T,<:HIPBanded{T}}}, B::ROCVecOrMat{T}) where {T<:Real} =
+    _gbqrm!('T', parent(Q), B)
+LinearAlgebra.rmul!(A::ROCMatrix{T}, Q::LinearAlgebra.QRPackedQ{T,<:HIPBanded{T}}) where {T} = _gbqrmr!('N', A, Q)
+LinearAlgebra.rmul!(A::ROCMatrix{T}, Q::LinearAlgebra.AdjointQ{T,<:LinearAlgebra.QRPackedQ{T,<:HIPBanded{T}}}) where {T<:Real} =
+    _gbqrmr!('T', A, parent(Q))
 BandedMatrices._banded_lmul!(α::Number, A::HIPBanded{T}, ::HIPBC) where {T} = _gbscal!(T(α), A)
 BandedMatrices._banded_rmul!(A::HIPBanded{T}, α::Number, ::HIPBC) where {T} = _gbscal!(T(α), A)
 BandedMatrices._banded_axpy!(a::Number, X::HIPBanded{T}, Y::HIPBanded{T}, ::HIPBC, ::HIPBC) where {T} =

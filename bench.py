@@ -46,6 +46,25 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def _host_threads():
+    """Host cores this process may run on (the affinity mask, not the machine's count)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def make_precision_problem(n=65536, l=128, u=128, nrhs=16, shift=True, seed=0):
     rng = np.random.default_rng(seed)
     A = np.asfortranarray(rng.standard_normal((l + u + 1, n)))
@@ -196,27 +215,30 @@ def bench_bandwidth(bh, steps, skip_cpu=True):
             ms = gpu_time(bh, one_t, max(30, steps * 6))
             out["gbmv_fp64_T_8x8_n2p20"] = {"GBps": nbytes / ms / 1e6, "frac_of_hbm_peak": nbytes / ms / 1e6 /
                                             HBM_PEAK_GBPS, "us": ms * 1e3, "bytes": nbytes}
-            # 64 right-hand sides: A read once, X and Y streamed
-            X = bh.DeviceArray.from_numpy(np.asfortranarray(rng.standard_normal((n, 64))))
-            Y = bh.DeviceArray((n, 64), np.float64)
-            A0 = sets[0][0]
-            f = lambda: bh.mul(Y, A0, X)
-            f()
-            bh.sync()
-            ms = gpu_time(bh, f, max(5, steps))
-            nbytes = (17 + 64 + 64) * 8.0 * n
-            out["gbmv_fp64_8x8_64rhs_n2p20"] = {"GBps": nbytes / ms / 1e6, "frac_of_hbm_peak": nbytes / ms / 1e6 / HBM_PEAK_GBPS,
-                                                "us": ms * 1e3, "bytes": nbytes}
-            # A*B: (8,8) x (4,4) -> (12,12)
-            Bm = bh.BandedMatrix.from_host(np.asfortranarray(rng.standard_normal((9, n))), n, 4, 4)
-            Cm = bh.BandedMatrix.zeros((n, n), (12, 12))
-            g = lambda: bh.gbmm(A0, Bm, Cm)
-            g()
-            bh.sync()
-            ms = gpu_time(bh, g, max(5, steps))
-            nbytes = (17 + 9 + 25) * 8.0 * n
-            out["gbmm_fp64_8x8_4x4_n2p20"] = {"GBps": nbytes / ms / 1e6, "frac_of_hbm_peak": nbytes / ms / 1e6 / HBM_PEAK_GBPS,
-                                              "us": ms * 1e3, "bytes": nbytes}
+        es = np.dtype(dt).itemsize
+        # 64 right-hand sides: A read once, X and Y streamed
+        X = bh.DeviceArray.from_numpy(np.asfortranarray(rng.standard_normal((n, 64)).astype(dt)))
+        Y = bh.DeviceArray((n, 64), dt)
+        A0 = sets[0][0]
+        f = lambda: bh.mul(Y, A0, X)
+        f()
+        bh.sync()
+        ms = gpu_time(bh, f, max(5, steps))
+        nbytes = (17 + 64 + 64) * float(es) * n
+        out[f"gbmv_{tag}_8x8_64rhs_n2p20"] = {"GBps": nbytes / ms / 1e6, "frac_of_hbm_peak": nbytes / ms / 1e6 /
+                                              HBM_PEAK_GBPS, "us": ms * 1e3, "bytes": nbytes}
+        del X, Y
+        # A*B: (8,8) x (4,4) -> (12,12)
+        Bm = bh.BandedMatrix.from_host(np.asfortranarray(rng.standard_normal((9, n)).astype(dt)), n, 4, 4)
+        Cm = bh.BandedMatrix.zeros((n, n), (12, 12), dt)
+        g = lambda: bh.gbmm(A0, Bm, Cm)
+        g()
+        bh.sync()
+        ms = gpu_time(bh, g, max(5, steps))
+        nbytes = (17 + 9 + 25) * float(es) * n
+        out[f"gbmm_{tag}_8x8_4x4_n2p20"] = {"GBps": nbytes / ms / 1e6, "frac_of_hbm_peak": nbytes / ms / 1e6 /
+                                            HBM_PEAK_GBPS, "us": ms * 1e3, "bytes": nbytes}
+        del Bm, Cm, sets
     if not skip_cpu:
         # the reference's CPU path: one dgbmv per right-hand side, gbmm! per column
         import scipy.linalg.blas as sb
@@ -236,21 +258,29 @@ def bench_bandwidth(bh, steps, skip_cpu=True):
                 us = (time.perf_counter() - t0) / 10 * 1e6
                 cpu[f"gbmv_fp64_us_{tag}"] = us
                 cpu[f"gbmv_fp64_GBps_{tag}"] = 152.0 * n / us / 1e3
-                cpu[f"gbmv_fp64_64rhs_us_{tag}"] = 64 * us  # 64 separate calls, A re-read each time
+                Xh = np.asfortranarray(rng.standard_normal((n, 64)))
+                t0 = time.perf_counter()
+                for c in range(64):  # the reference's mul! loop: one gbmv per column
+                    sb.dgbmv(n, n, 8, 8, 1.0, Ah, Xh[:, c])
+                cpu[f"gbmv_fp64_64rhs_us_{tag}"] = (time.perf_counter() - t0) * 1e6
         Bh = np.asfortranarray(rng.standard_normal((9, n)))
         t0 = time.perf_counter()
         O.gbmm(n, n, n, 1.0, Ah, 8, 8, Bh, 4, 4, 0.0, np.zeros((25, n), order="F"), 12, 12)
         cpu["gbmm_fp64_us_1core"] = (time.perf_counter() - t0) * 1e6
         cpu["cores"] = os.cpu_count()
-        cpu["sample"] = ("OpenBLAS 0.3.28 (scipy) dgbmv, mean of 10 calls per thread count; 64 RHS as 64 "
-                         "calls (what the reference's per-column mul! loop does); A*B by the C restatement of gbmm!")
+        cpu["sample"] = ("OpenBLAS 0.3.28 (scipy) dgbmv, mean of 10 calls per thread count; 64 RHS timed as the 64 "
+                         "per-column calls the reference's mul! loop makes; A*B by the C restatement of gbmm!")
         out["cpu_bandwidth_n2p20"] = cpu
     return out
 
 
 # ----------------------------------------------------------------- batching
 def bench_batched(bh, skip_cpu):
+    """4096 systems in one graph launch.  Algorithmic bytes per system: the
+    padded band (2l+u+1) x n read and written once, the right-hand side read
+    and the solution written: itemsize * n * (2(2l+u+1) + 2)."""
     n, kl, ku, bands, rhs = bh.c4_batch(4096, 0)
+    slots = float(np.sum(n * (2 * (2 * kl + ku + 1) + 2)))
     out = {"systems": 4096, "rows_total": int(n.sum())}
     for dt, tag in ((np.float64, "fp64"), (np.float32, "fp32")):
         bt = bh.BatchedGBSV(n, kl, ku, dt)
@@ -260,7 +290,7 @@ def bench_batched(bh, skip_cpu):
         g = bt.graph()
         ms = []
         t = bh.Timer()
-        for _ in range(3):
+        for _ in range(5):
             bt.upload(AB, B)
             bh.sync()
             t.start()
@@ -268,28 +298,43 @@ def bench_batched(bh, skip_cpu):
             t.stop()
             ms.append(t.elapsed_ms())
         g.close()
-        out[tag] = {"ms_per_batch": float(np.median(ms)), "systems_per_s": 4096 / (np.median(ms) * 1e-3),
-                    "schedule": list(bt.schedule()), "autotune_best_ms": tuned_ms}
+        med = float(np.median(ms))
+        nbytes = slots * np.dtype(dt).itemsize
+        out[tag] = {"ms_per_batch": med, "systems_per_s": 4096 / (med * 1e-3), "schedule": list(bt.schedule()),
+                    "autotune_best_ms": tuned_ms,
+                    "roofline": {"bound": "hbm", "achieved": nbytes / (med * 1e-3) / 1e9, "peak": HBM_PEAK_GBPS,
+                                 "unit": "GB/s", "frac": nbytes / (med * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+                                 "bytes_per_launch": nbytes}}
         bt.close()
     if not skip_cpu:
         import scipy.linalg.lapack as la
+        from threadpoolctl import threadpool_limits
 
-        sample = range(0, 4096, 16)  # 256 systems, scaled to the batch
-        t0 = time.perf_counter()
-        for s in sample:
-            l, u = int(kl[s]), int(ku[s])
-            ab = np.zeros((2 * l + u + 1, n[s]), order="F")
-            ab[l:, :] = bands[s]
-            lub, piv, info = la.dgbtrf(ab, l, u)
-            la.dgbtrs(lub, l, u, rhs[s], piv)
-        tc = (time.perf_counter() - t0) * 4096 / len(sample)
-        out["cpu_1core_ms_per_batch_est"] = tc * 1e3
-        out["cpu_sample"] = "every 16th system (256) with scipy OpenBLAS dgbtrf+dgbtrs on 1 core, scaled x16"
+        with threadpool_limits(1):
+            t0 = time.perf_counter()
+            for s in range(4096):
+                l, u = int(kl[s]), int(ku[s])
+                ab = np.zeros((2 * l + u + 1, n[s]), order="F")
+                ab[l:, :] = bands[s]
+                lub, piv, info = la.dgbtrf(ab, l, u)
+                la.dgbtrs(lub, l, u, rhs[s], piv)
+            out["cpu_1core_ms_per_batch"] = (time.perf_counter() - t0) * 1e3
+        out["cpu_sample"] = "all 4096 systems, scipy OpenBLAS dgbtrf+dgbtrs one after another on 1 thread, FP64"
     return out
 
 
 # ----------------------------------------------------------------- differentiation
 def bench_adjoint(bh, steps, skip_cpu):
+    """Gradient of dot(w, A\\b) + 0.5||A x0||^2 at n = 262144, l = u = 16.
+    Each piece is timed on its own beside the whole; algorithmic bytes per
+    piece (FP64): LU reads the band and writes the padded factors
+    ((l+u+1) + (2l+u+1)) 8n; each solve reads the factors and one vector and
+    writes one ((2l+u+1) + 2) 8n; gbmv (l+u+1+2) 8n; the band-projected update
+    reads four vectors and writes the band ((l+u+1) + 4) 8n."""
+    import ctypes as C
+
+    from banded_hip import _lib
+
     n, l, u = 262144, 16, 16
     rng = np.random.default_rng(4)
     A = np.asfortranarray(rng.standard_normal((l + u + 1, n)))
@@ -301,16 +346,100 @@ def bench_adjoint(bh, steps, skip_cpu):
     bh.sync()
     ms = gpu_time(bh, lambda: bh.loss_and_grad(dA, db, dw, dx0), max(2, steps // 2))
     out = {"n": n, "l": l, "u": u, "ms_loss_and_grad": ms}
+    # the pieces
+    F = bh.lu(dA)
+    x = db.copy()
+    lam = dw.copy()
+    y0 = bh.DeviceArray((n,), np.float64)
+    G = bh.BandedMatrix.zeros((n, n), (l, u))
+    hp = 2 * l + u + 1
+    pieces = {
+        "lu": (lambda: bh.lu(dA), ((l + u + 1) + hp) * 8.0 * n),
+        "ldiv_N": (lambda: bh.ldiv_(F, x), (hp + 2) * 8.0 * n),
+        "gbmv": (lambda: bh.mul(y0, dA, dx0), (l + u + 3) * 8.0 * n),
+        "ldiv_T": (lambda: bh.ldiv_(F.transpose(), lam), (hp + 2) * 8.0 * n),
+        "band_grad": (lambda: _lib.call("bh_dgb_band_grad", n, n, l, u, lam.ptr, x.ptr, y0.ptr, dx0.ptr, G.data.ptr,
+                                        G.data.ld), (l + u + 1 + 4) * 8.0 * n),
+    }
+    per = {}
+    for name, (fn, nbytes) in pieces.items():
+        fn()
+        bh.sync()
+        pm = gpu_time(bh, fn, 3)
+        per[name] = {"ms": pm, "bytes": nbytes, "GBps": nbytes / (pm * 1e-3) / 1e9,
+                     "frac_of_hbm_peak": nbytes / (pm * 1e-3) / 1e9 / HBM_PEAK_GBPS}
+    out["pieces"] = per
+    big = max(per, key=lambda k: per[k]["ms"])
+    out["roofline"] = {"kernel": f"{big} (the longest piece)", "bound": "hbm", "achieved": per[big]["GBps"],
+                       "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": per[big]["frac_of_hbm_peak"],
+                       "bytes_per_launch": per[big]["bytes"]}
     if not skip_cpu:
+        import scipy.linalg.blas as sb
         import scipy.linalg.lapack as la
+        from threadpoolctl import threadpool_limits
 
-        t0 = time.perf_counter()
-        ab = np.zeros((2 * l + u + 1, n), order="F")
-        ab[l:, :] = A
-        lub, piv, info = la.dgbtrf(ab, l, u)
-        x, _ = la.dgbtrs(lub, l, u, b, piv)
-        lam, _ = la.dgbtrs(lub, l, u, w, piv, trans=1)
-        out["cpu_ms_loss_and_grad_lapack_part"] = (time.perf_counter() - t0) * 1e3
+        def cpu_grad():
+            ab = np.zeros((hp, n), order="F")
+            ab[l:, :] = A
+            lub, piv, info = la.dgbtrf(ab, l, u)
+            xc, _ = la.dgbtrs(lub, l, u, b, piv)
+            yc = sb.dgbmv(n, n, l, u, 1.0, A, x0)
+            lc, _ = la.dgbtrs(lub, l, u, w, piv, trans=1)
+            gA = np.empty_like(A)
+            for t in range(l + u + 1):  # row t of the band holds A[j - u + t, j]
+                j0, j1 = max(0, u - t), min(n, n + u - t)
+                i0 = j0 - u + t
+                i1 = i0 + (j1 - j0)
+                gA[t, j0:j1] = yc[i0:i1] * x0[j0:j1] - lc[i0:i1] * xc[j0:j1]
+            return float(w @ xc + 0.5 * yc @ yc), gA, lc
+
+        for thr, tag in ((1, "1core"), (_host_threads(), "allcores")):
+            with threadpool_limits(thr):
+                cpu_grad()
+                t0 = time.perf_counter()
+                for _ in range(3):
+                    cpu_grad()
+                out[f"cpu_ms_loss_and_grad_{tag}"] = (time.perf_counter() - t0) / 3 * 1e3
+        out["cpu_sample"] = ("full reference gradient: OpenBLAS dgbtrf + dgbtrs 'N' + dgbmv + dgbtrs 'T' + the band "
+                             "projection (numpy, one vector op per band row); mean of 3 after 1 warm-up")
+    return out
+
+
+# ----------------------------------------------------------------- banded QR
+def bench_qr(bh, steps, skip_cpu):
+    """qr(A) and qr(A) \\ b at the reference's benchmark shapes
+    (test/benchmark.jl:131-148): n = 100000 with (3,4) and n = 2000 with (400,400)."""
+    out = {}
+    for n, l, u in ((100000, 3, 4), (2000, 400, 400)):
+        rng = np.random.default_rng(5)
+        A_h = np.asfortranarray(rng.random((l + u + 1, n)))
+        b_h = rng.standard_normal(n)
+        A = bh.BandedMatrix.from_host(A_h, n, l, u)
+        b = bh.DeviceArray.from_numpy(b_h)
+        F = bh.qr(A)
+        bh.sync()
+        r = {"n": n, "l": l, "u": u}
+        r["ms_qr"] = gpu_time(bh, lambda: bh.qr(A), max(2, steps // 2))
+        r["ms_qr_ldiv"] = gpu_time(bh, lambda: bh.qr_ldiv_(F, b.copy()), max(2, steps // 2))
+        x = bh.qr_ldiv_(F, b.copy()).numpy()
+        import scipy.linalg.blas as sb
+
+        res = b_h - sb.dgbmv(n, n, l, u, 1.0, A_h, x)
+        r["relative_residual"] = float(np.linalg.norm(res) / np.linalg.norm(b_h))
+        if not skip_cpu:
+            from oracle import oracle as O
+
+            ab = np.zeros((2 * l + u + 1, n), order="F")
+            ab[l:, :] = A_h
+            t0 = time.perf_counter()
+            fac, tau = O.gbqrf(n, n, l, l + u, ab.copy(order="F"))
+            r["cpu_ms_qr_1core"] = (time.perf_counter() - t0) * 1e3
+            t0 = time.perf_counter()
+            O.gbqrs(n, n, l, l + u, fac, tau, b_h.copy())
+            r["cpu_ms_qr_ldiv_1core"] = (time.perf_counter() - t0) * 1e3
+            r["cpu_sample"] = ("the C restatement of the reference's banded_qr! / lmul! + tbsv (bandedqr.jl), "
+                               "1 thread, one call each")
+        out[f"qr_n{n}_{l}x{u}"] = r
     return out
 
 
@@ -623,12 +752,14 @@ def main():
         extra["elementwise_n2p20_8x8"] = bench_elementwise(bh, args.steps, args.skip_cpu)
         log("symmetric band Cholesky and product")
         extra.update(bench_cholesky(bh, args.steps, args.skip_cpu))
+        log("banded QR")
+        extra.update(bench_qr(bh, args.steps, args.skip_cpu))
         log("precision workload, no diagonal shift (pivoting active)")
         P2 = bench_precision(bh, max(1, args.steps // 2), 1, shift=False)
         ns = {"mixed_solves_per_s": 1000.0 / P2["ms_mixed"], "fp64_gpu_solves_per_s": 1000.0 / P2["ms_fp64"],
               "refinement_iters": P2["iters"]}
         if not args.skip_cpu:
-            xc2, pivc2, tc2 = cpu_reference_solve(P2["A_h"], P2["B_h"], P2["l"], P2["u"], os.cpu_count() or 1)
+            xc2, pivc2, tc2 = cpu_reference_solve(P2["A_h"], P2["B_h"], P2["l"], P2["u"], _host_threads())
             fe2 = lambda x: float(np.abs(x - xc2).max() / np.abs(xc2).max())
             ns.update({"cpu_solves_per_s": 1.0 / tc2, "forward_error_mixed": fe2(P2["x_mixed"]),
                        "forward_error_fp64_gpu": fe2(P2["x_fp64"]),
@@ -640,11 +771,25 @@ def main():
     out = {}
     if not args.skip_cpu:
         log("CPU baseline")
-        threads = args.cpu_threads or os.cpu_count() or 1
-        tcpu = []
-        for _ in range(3):
-            xc, pivc, tc = cpu_reference_solve(P["A_h"], P["B_h"], l, u, threads)
-            tcpu.append(tc)
+        allt = args.cpu_threads or _host_threads()
+        per = {}
+        for thr in sorted({1, allt}):
+            for _ in range(2):  # warm-up
+                cpu_reference_solve(P["A_h"], P["B_h"], l, u, thr)
+            ts = []
+            for _ in range(10):
+                xc, pivc, tc = cpu_reference_solve(P["A_h"], P["B_h"], l, u, thr)
+                ts.append(tc)
+            per[thr] = float(np.median(ts))
+        best = min(per, key=per.get)
+        out["cpu_baseline"] = {
+            "value": 1.0 / per[best], "unit": "solves/s", "cores": best, "kind": "reference",
+            "sample": ("the whole workload (n=65536, l=u=128, 16 RHS), FP64, OpenBLAS 0.3.28 dgbtrf+dgbtrs via "
+                       "scipy -- the LAPACK routines BandedLU.jl:98 / linalg.jl:28 call; median of 10 after 2 "
+                       "warm-ups, at 1 thread and at every host thread; value = the faster"),
+            "solves_per_s_by_threads": {str(k): 1.0 / v for k, v in per.items()},
+            "cpu_model": _cpu_model(), "nproc": _host_threads(),
+        }
         fe = lambda x: float(np.abs(x - xc).max() / np.abs(xc).max())
         out["cpu_baseline"] = {
             "value": 1.0 / float(np.median(tcpu)), "unit": "solves/s", "cores": threads, "kind": "reference",
@@ -687,7 +832,8 @@ def main():
         "refinement_iters": P["iters"],
         "gbmv_GBps": g["GBps"],
         "roofline": {
-            "kernel": "gbtrf_blk_kernel<float,15,10,4> (FP32 band LU, blocked; the largest kernel of the mixed A\\b)",
+            "kernel": "gbtrf_mfma_kernel<float,3> (FP32 blocked band LU, MFMA trailing updates; the largest "
+                      "kernel of the mixed A\\b); kernel_ms is the bh_sgbtrf call (zero-fill pre-pass + LU)",
             "bound": "mfma", "achieved": lu_tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": lu_tf / FP32_PEAK_TFLOPS, "traffic": traffic, "kernel_ms": P["lu32_ms"],
             "flops_per_launch": lu_fl,

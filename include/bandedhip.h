@@ -37,6 +37,7 @@ typedef int64_t bh_int;
 #define BH_ERR_BANDERROR (-1003)  /* C lacks bands for A*B (BandError, Band.jl:75-88) */
 #define BH_ERR_WORKSPACE (-1004)  /* workspace too small */
 #define BH_ERR_INTERNAL (-1005)   /* a kernel's bounded internal wait gave up (results invalid) */
+#define BH_ERR_BUSY (-1006)       /* a batch plan still has captured graphs (schedule fixed, plan kept) */
 
 /* ---------------------------------------------------------------- runtime */
 /* No reference counterpart: device/memory plumbing the Julia extension uses to
@@ -153,6 +154,14 @@ int bh_dgbqrm(char trans, bh_int m, bh_int n, bh_int l, bh_int u, const double *
               const double *tau, bh_int nrhs, double *B, bh_int ldb);
 int bh_sgbqrm(char trans, bh_int m, bh_int n, bh_int l, bh_int u, const float *D, bh_int ldd,
               const float *tau, bh_int nrhs, float *B, bh_int ldb);
+/* A <- A Q (trans 'N') or A Q' ('T' / 'C'); A is mA x mQ (lda >= mA), Q the
+ * orthogonal factor of the mQ x nQ band QR (D, tau as bh_?gbqrf returns them).
+ * Replaces banded_qr_rmul! (src/banded/bandedqr.jl:96-148) behind rmul!(A, Q)
+ * and rmul!(A, Q') (bandedqr.jl:210-214). */
+int bh_dgbqrmr(char trans, bh_int mA, bh_int mQ, bh_int nQ, bh_int l, bh_int u, const double *D,
+               bh_int ldd, const double *tau, double *A, bh_int lda);
+int bh_sgbqrmr(char trans, bh_int mA, bh_int mQ, bh_int nQ, bh_int l, bh_int u, const float *D,
+               bh_int ldd, const float *tau, float *A, bh_int lda);
 /* Least-squares / square solve with the factors: B (m x nrhs, m >= n) <- Q'B,
  * then the first n rows <- R \ (Q'B)[1:n] (upper band solve, u super-diagonals).
  * m < n is an argument error (-1), as the reference raises for wide QR.

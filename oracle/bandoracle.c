@@ -595,6 +595,26 @@ void bo_dgbqr_qmul(i64 m, i64 n, i64 l, i64 u, const double *D, i64 ldd, const d
     }
 }
 
+/* banded_qr_rmul! (bandedqr.jl:96-148): A <- A Q (trans 'N', reflectors in
+ * ascending order) or A Q' ('T', descending); A is mA x mQ dense, Q comes from
+ * the mQ x nQ factors D.  Per row: v = A(i,k) + sum_j A(i,j) v_j, v *= tau_k,
+ * A(i,k) -= v, A(i,j) -= v v_j for j = k+1 .. min(k+l, mQ-1). */
+void bo_dgbqr_rmul(char trans, i64 mA, i64 mQ, i64 nQ, i64 l, i64 u, const double *D, i64 ldd,
+                   const double *tau, double *A, i64 lda) {
+    i64 kmax = imin(mQ, nQ);
+    for (i64 t = 0; t < kmax; ++t) {
+        i64 k = (trans == 'N' || trans == 'n') ? t : kmax - 1 - t;
+        i64 jhi = imin(k + l, mQ - 1);
+        for (i64 i = 0; i < mA; ++i) {
+            double v = A[IDX(lda, i, k)];
+            for (i64 j = k + 1; j <= jhi; ++j) v += A[IDX(lda, i, j)] * D[IDX(ldd, j - k + u, k)];
+            v = v * tau[k];
+            A[IDX(lda, i, k)] -= v;
+            for (i64 j = k + 1; j <= jhi; ++j) A[IDX(lda, i, j)] -= v * D[IDX(ldd, j - k + u, k)];
+        }
+    }
+}
+
 /* ldiv!(QR, B) for m >= n (bandedqr.jl:178-191): B <- Q'B, then
  * R[1:n,1:n] \ B[1:n,:] with the tbsv upper solve of bandwidth u. */
 void bo_dgbqrs(i64 m, i64 n, i64 l, i64 u, const double *D, i64 ldd, const double *tau,

@@ -81,6 +81,8 @@ def lib():
         for nm in ("bo_dgbqr_qtmul", "bo_dgbqr_qmul"):
             getattr(L, nm).argtypes = [i64, i64, i64, i64, _dp, i64, _dp, i64, i64, _dp, i64]
             getattr(L, nm).restype = None
+        L.bo_dgbqr_rmul.argtypes = [C.c_char, i64, i64, i64, i64, i64, _dp, i64, _dp, _dp, i64]
+        L.bo_dgbqr_rmul.restype = None
         L.bo_dgbqrs.argtypes = [i64, i64, i64, i64, _dp, i64, _dp, i64, _dp, i64]
         L.bo_dgbqrs.restype = None
         L.bo_dgb_band_grad.argtypes = [i64, i64, i64, _dp, _dp, _dp, _dp, _dp, i64]
@@ -256,6 +258,16 @@ def gbqr_qmul(m, n, l, u, D, tau, B):
     lib().bo_dgbqr_qmul(m, n, l, u, _ptr(d, _dp), d.shape[0], _ptr(t, _dp), len(t), bb.shape[1],
                         _ptr(bb, _dp), max(m, 1))
     return bb[:, 0].copy() if vec else bb
+
+
+def gbqr_rmul(trans, mQ, nQ, l, u, D, tau, A):
+    """A (mA x mQ) <- A Q ('N') or A Q' ('T'), banded_qr_rmul! restated."""
+    d = _f(D, np.float64)
+    t = np.ascontiguousarray(tau, np.float64)
+    aa = _f(A, np.float64).copy(order="F")
+    lib().bo_dgbqr_rmul(trans.encode(), aa.shape[0], mQ, nQ, l, u, _ptr(d, _dp), d.shape[0], _ptr(t, _dp),
+                        _ptr(aa, _dp), max(aa.shape[0], 1))
+    return aa
 
 
 def gbqrs(m, n, l, u, D, tau, B):

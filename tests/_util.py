@@ -22,13 +22,8 @@ def in_band_mask(m, n, l, u, hrows=None, shift=0):
     """Boolean mask over a band array marking slots inside the m x n matrix.
     shift = number of extra rows above (u already includes them)."""
     h = hrows if hrows is not None else l + u + 1
-    mask = np.zeros((h, n), bool)
-    for j in range(n):
-        for t in range(h):
-            i = j - u + t
-            if 0 <= i < m:
-                mask[t, j] = True
-    return mask
+    i = np.arange(n)[None, :] - u + np.arange(h)[:, None]  # row of slot (t, j)
+    return (i >= 0) & (i < m)
 
 
 def bits_equal(a, b):

@@ -125,6 +125,13 @@ def test_gpu_lu_golden(bh, tag):
         B = bh.DeviceArray.from_numpy(z[f"{tag}_B"].copy())
         bh.ldiv_(F, B)
         assert bits_equal(B.numpy(), z[f"{tag}_xN"])
+    if f"{tag}_xT" in z:
+        # the transposed solve on LAPACK's factors, row interchanges included
+        ref = z[f"{tag}_xT"]
+        B = bh.DeviceArray.from_numpy(z[f"{tag}_B"].copy())
+        bh.ldiv_(F.transpose(), B)
+        tol = 1e-13 if ref.dtype == np.float64 else 1e-5
+        assert np.abs(B.numpy() - ref).max() <= tol * (np.abs(ref).max() + 1)
 
 
 @pytest.mark.gpu

@@ -98,3 +98,30 @@ def test_batched_plan_rejects_wide_kl(bh):
 
     with pytest.raises(BandedHIPError):
         bh.BatchedGBSV([100, 100], [2, 9], [2, 2])
+
+
+def test_batched_schedule_is_fixed_while_a_graph_exists(bh, oracle):
+    """A captured graph bakes in the schedule: changing it (or autotuning)
+    while the graph exists is an error (BH_ERR_BUSY), and works again once
+    the graph is destroyed; the graph keeps its plan and buffers alive."""
+    from banded_hip import BandedHIPError
+
+    n, kl, ku, bands, rhs = _batch(bh, np.float64, nsys=16, seed=4)
+    bt = bh.BatchedGBSV(n, kl, ku)
+    AB, B = bt.pack(bands, rhs)
+    bt.upload(AB, B)
+    g = bt.graph()
+    with pytest.raises(BandedHIPError) as ei:
+        bt.schedule(order=1, waves_per_block=1, blocks=8)
+    assert ei.value.status == -1006
+    with pytest.raises(BandedHIPError):
+        bt.autotune()
+    g.launch()
+    bh.sync()
+    _check(oracle, bt, n, kl, ku, AB, B)
+    g.close()
+    bt.schedule(order=1, waves_per_block=1, blocks=8)
+    bt.upload(AB, B)
+    bt.solve()
+    _check(oracle, bt, n, kl, ku, AB, B)
+    bt.close()

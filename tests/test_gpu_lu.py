@@ -156,3 +156,62 @@ def test_zero_size(bh):
     F = bh.lu(A)
     assert F.ipiv.size == 0 and F.info == 0
     assert bh.ldiv_(F, bh.DeviceArray((0,))).shape == (0,)
+
+
+PIV_CASES = [
+    # n, kl, ku: no diagonal shift, N(0,1) entries -- row interchanges on most steps
+    (300, 4, 3),
+    (2000, 8, 8),
+    (4096, 16, 16),
+    (600, 24, 24),
+    (260, 64, 64),
+    (1000, 128, 128),  # blocked MFMA factorisation (ku >= 65)
+]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n,kl,ku", PIV_CASES)
+@pytest.mark.parametrize("nrhs", [1, 3])
+def test_gbtrs_with_row_interchanges(bh, oracle, dtype, n, kl, ku, nrhs):
+    """gbtrs 'N' and 'T' on factors with pivoting active.  Both solves run on the
+    GPU's own factors, and so does the oracle's xGBTRS restatement: 'N' must be
+    bit-identical (the same recipe as dgbtrs), 'T' within 1e-11 (FP64) / 1e-3
+    (FP32) of the oracle relative to the solution's scale."""
+    rng = np.random.default_rng(n + 13 * kl + nrhs)
+    A = band(rng, kl, ku, n, np.float64, dist="n").astype(dtype)
+    F = bh.lu(bh.BandedMatrix.from_host(A, n, kl, ku), check=False)
+    assert F.info == 0
+    swaps = int(np.sum(F.ipiv != np.arange(1, n + 1)))
+    assert swaps > n // 4, swaps
+    f = F.factors.host_data()
+    B = np.asfortranarray(rng.standard_normal((n, nrhs)).astype(dtype))
+    for trans in ("N", "T"):
+        ref = oracle.gbtrs(trans, n, kl, ku, f, F.ipiv, B)
+        dB = bh.DeviceArray.from_numpy(B)
+        bh.ldiv_(F if trans == "N" else F.transpose(), dB)
+        out = dB.numpy()
+        if trans == "N":
+            assert bits_equal(out, ref), np.abs(out - ref).max()
+        else:
+            tol = 1e-11 if dtype == np.float64 else 1e-3
+            assert np.abs(out - ref).max() <= tol * (np.abs(ref).max() + 1)
+
+
+def test_ldiv_into_a_column_view(bh, oracle):
+    """ldiv! on a 3-column window of a larger array solves exactly those columns
+    in place and leaves the rest untouched (the leading dimension is the parent's)."""
+    rng = np.random.default_rng(21)
+    n, kl, ku = 300, 5, 4
+    A = band(rng, kl, ku, n, dist="n")
+    F = bh.lu(bh.BandedMatrix.from_host(A, n, kl, ku))
+    big = np.asfortranarray(rng.standard_normal((n + 7, 6)))
+    dbig = bh.DeviceArray.from_numpy(big)
+    view = dbig.view(2, 1, (n, 3))
+    bh.ldiv_(F, view)
+    out = dbig.numpy()
+    f = F.factors.host_data()
+    ref = oracle.gbtrs("N", n, kl, ku, f, F.ipiv, np.asfortranarray(big[2:2 + n, 1:4]))
+    assert bits_equal(np.ascontiguousarray(out[2:2 + n, 1:4]), np.ascontiguousarray(ref))
+    mask = np.ones(big.shape, bool)
+    mask[2:2 + n, 1:4] = False
+    assert np.array_equal(out[mask], big[mask])

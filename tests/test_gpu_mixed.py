@@ -51,7 +51,7 @@ def test_mixed_fp32_factor_bits(bh, oracle):
     assert np.array_equal(F.ipiv, piv) and bits_equal(F.factors.host_data(), f)
 
 
-@pytest.mark.parametrize("n,kl,ku", [(2048, 64, 64), (3000, 16, 16)])
+@pytest.mark.parametrize("n,kl,ku", [(2048, 64, 64), (3000, 16, 16), (4096, 128, 128)])
 def test_mixed_unshifted_converges_or_falls_back(bh, oracle, n, kl, ku):
     A, B = _problem(7 + n, n, kl, ku, 4, 0.0)
     dA = bh.BandedMatrix.from_host(A, n, kl, ku)
@@ -79,3 +79,22 @@ def test_mixed_workspace_too_small_is_an_error(bh):
     st = L.bh_dsgbsv(n, kl, ku, 1, dA.data.ptr, dA.data.ld, dB.ptr, n, dX.ptr, n, None, 0, None, C.byref(it),
                      C.byref(info))
     assert st == -1004
+
+
+def test_mixed_unshifted_128_forward_error(bh, oracle):
+    """The precision workload's pivoting variant at reduced n: l = u = 128, no
+    diagonal shift, 16 right-hand sides.  The refined FP64 solution agrees with
+    the FP64 oracle's solve to 1e-12 (forward error relative to the solution's
+    max norm), whether refinement converged (iters >= 0) or fell back to the
+    FP64 factorisation (iters < 0); row interchanges are active."""
+    n, kl, ku, nrhs = 4096, 128, 128, 16
+    A, B = _problem(99, n, kl, ku, nrhs, 0.0)
+    dA = bh.BandedMatrix.from_host(A, n, kl, ku)
+    res = bh.solve_mixed(dA, bh.DeviceArray.from_numpy(B))
+    f, piv, info = oracle.gbtf2(n, n, kl, ku, oracle.pad_for_lu(A, n, n, kl, ku))
+    assert info == 0
+    assert int(np.sum(piv != np.arange(1, n + 1))) > n // 2
+    xref = oracle.gbtrs("N", n, kl, ku, f, piv, B)
+    x = res.x.numpy()
+    ferr = np.abs(x - xref).max() / np.abs(xref).max()
+    assert ferr <= 1e-12, (res.iters, ferr)

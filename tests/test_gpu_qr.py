@@ -105,3 +105,55 @@ def test_qr_wide_solve_is_an_error(bh):
     F = bh.qr(bh.BandedMatrix.from_host(A, 30, 2, 2))
     with pytest.raises(ValueError):
         bh.qr_ldiv_(F, bh.DeviceArray.from_numpy(np.ones(30)))
+
+
+RMUL_CASES = [
+    # mA, m, n, l, u
+    (7, 300, 300, 4, 3),
+    (50, 400, 300, 3, 5),
+    (33, 300, 400, 3, 5),
+    (20, 300, 300, 40, 10),
+    (9, 200, 200, 70, 10),  # l + 1 > 64
+]
+
+
+@pytest.mark.parametrize("mA,m,n,l,u", RMUL_CASES)
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_q_rmul(bh, oracle, mA, m, n, l, u, trans):
+    """A Q and A Q' (banded_qr_rmul!): bit-identical to the oracle's restatement
+    (the same sequential sums per row), and equal to the dense product."""
+    rng = np.random.default_rng(mA + m + 7 * l + u)
+    A = _make(rng, m, n, l, u)
+    F = bh.qr(bh.BandedMatrix.from_host(A, m, l, u))
+    D = F.factors.host_data()
+    tau = F.tau.numpy()[: min(m, n)]
+    X = np.asfortranarray(rng.standard_normal((mA, m)))
+    ref = oracle.gbqr_rmul(trans, m, n, l, l + u, D, tau, X)
+    dX = bh.DeviceArray.from_numpy(X)
+    Q = F.Q if trans == "N" else F.Q.T
+    bh.qr_rmul_(dX, Q)
+    out = dX.numpy()
+    assert bits_equal(out, ref), np.abs(out - ref).max()
+    Qd = oracle.gbqr_qmul(m, n, l, l + u, D, tau, np.eye(m))
+    dense = X @ (Qd if trans == "N" else Qd.T)
+    assert np.abs(out - dense).max() <= 1e-12 * (np.abs(dense).max() + 1)
+
+
+@pytest.mark.parametrize("n,l,u", [(2000, 400, 400), (4096, 128, 128), (1500, 200, 150)])
+def test_qr_solve_wide_bands(bh, oracle, n, l, u):
+    """qr(A) \\ b where the R band (l+u super-diagonals) is beyond the lane-ring
+    solve: the reference's own benchmark shape (2000, (400,400)) among them."""
+    rng = np.random.default_rng(n + l)
+    A = _make(rng, n, n, l, u)
+    B = np.asfortranarray(rng.standard_normal((n, 2)))
+    F = bh.qr(bh.BandedMatrix.from_host(A, n, l, u))
+    dB = bh.DeviceArray.from_numpy(B)
+    bh.qr_ldiv_(F, dB)
+    x = dB.numpy()
+    D, tau = oracle.gbqrf(n, n, l, l + u, _padded(oracle, A, n, n, l, u))
+    xref = oracle.gbqrs(n, n, l, l + u, D, tau, B)
+    assert np.abs(x - xref).max() <= 1e-10 * (np.abs(xref).max() + 1)
+    for c in range(2):
+        r = B[:, c] - oracle.gbmv("N", n, n, l, u, 1.0, A, x[:, c])
+        berr = np.abs(r).max() / (np.abs(A).sum(axis=0).max() * np.abs(x[:, c]).max() + np.abs(B[:, c]).max())
+        assert berr <= 1e-13, berr

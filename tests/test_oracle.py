@@ -308,6 +308,25 @@ def test_qr_q_orthogonal_and_solve(oracle):
     np.testing.assert_allclose(M @ x, b, atol=1e-11)
 
 
+@pytest.mark.parametrize("m,n", [(60, 60), (70, 60), (60, 70)])
+def test_qr_rmul_matches_dense_q(oracle, m, n):
+    """banded_qr_rmul! restated: A Q and A Q' equal the dense products with the
+    orthogonal factor built column by column from Q e_j (itself pinned by
+    QᵀQ = I and QR = A above); the two directions differ, so an inverted loop
+    order or a transposed application fails."""
+    rng = np.random.default_rng(m + 3 * n)
+    l, u = 3, 4
+    A = band(rng, l, u, n, m=m)
+    F, tau = oracle.gbqrf(m, n, l, l + u, oracle.pad_for_lu(A, m, n, l, u))
+    Q = oracle.gbqr_qmul(m, n, l, l + u, F, tau, np.eye(m))
+    X = rng.standard_normal((9, m))
+    XQ = oracle.gbqr_rmul("N", m, n, l, l + u, F, tau, X)
+    XQt = oracle.gbqr_rmul("T", m, n, l, l + u, F, tau, X)
+    np.testing.assert_allclose(XQ, X @ Q, atol=1e-13)
+    np.testing.assert_allclose(XQt, X @ Q.T, atol=1e-13)
+    assert np.abs(XQ - XQt).max() > 1e-3
+
+
 def test_qr_reflector_convention_on_triangular_column(oracle):
     """reflector! on x = [x1; 0...] returns tau = 2 and x1 -> -x1 (its
     convention; LAPACK's dlarfg would return tau = 0 here)."""
