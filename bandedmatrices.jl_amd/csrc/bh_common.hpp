@@ -25,6 +25,9 @@ hipStream_t stream();
 // the library stream).
 int ctrl_words(i64 **p);
 int scratch_buffer(size_t bytes, void **p);
+// Looks up, with `device` current, the device address of the statically
+// allocated scratch used by bh_dsgbsv_noalloc (once per device).
+int noalloc_setup(int device);
 void set_error(const char *fmt, ...);
 int hip_fail(hipError_t e, const char *what, const char *file, int line);
 

@@ -16,6 +16,7 @@
 // Residuals, norms and the solution are FP64 throughout.
 #include <math.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "bh_common.hpp"
@@ -116,6 +117,41 @@ static MixedWs mixed_layout(i64 n, i64 kl, i64 ku, i64 nrhs, void *base) {
 
 static dim3 grid1(i64 cnt) { return dim3((unsigned)cdiv(cnt, 256)); }
 
+// Bytes of the FP64 fallback (padded factors + pivots).
+static size_t fallback_bytes(i64 n, i64 kl, i64 ku) {
+    return align256((size_t)(2 * kl + ku + 1) * n * sizeof(double)) + (size_t)imax(n, 1) * sizeof(i64);
+}
+
+// ---- statically allocated scratch for bh_dsgbsv_noalloc.  The array lives in
+// this code object on every device that loads it; its device address is
+// looked up once per device by noalloc_setup (called from bh_init with the
+// device current) and only that address is ever handed to kernels or copies.
+constexpr size_t kNoAllocBytes = size_t(64) << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+static void *g_noalloc_ptr[kMaxDevices] = {};
+static int g_noalloc_status[kMaxDevices] = {};
+
+int noalloc_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return BH_OK;  // the entry point reports it
+    if (g_noalloc_ptr[device]) return BH_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+        // recorded, not fatal for bh_init: only bh_dsgbsv_noalloc needs it
+        g_noalloc_status[device] = e != hipSuccess ? (int)e : (int)hipErrorInvalidSymbol;
+        return BH_ERR_HIP;
+    }
+    g_noalloc_ptr[device] = p;
+    return BH_OK;
+}
+
+// The mixed solve with its workspace at `work`; the FP64 fallback takes its
+// factors from `fb` (nullptr: the library's growable scratch buffer).
+static int dsgbsv_impl(i64 n, i64 kl, i64 ku, i64 nrhs, const double *A, i64 lda, const double *B, i64 ldb,
+                       double *X, i64 ldx, void *work, size_t lwork, void *fb, i64 *ipiv32, i64 *iters_host,
+                       i64 *info_host);
+
 }  // namespace bh
 
 using namespace bh;
@@ -134,6 +170,50 @@ int bh_dsgbsv_workspace(bh_int n, bh_int kl, bh_int ku, bh_int nrhs, size_t *byt
 
 int bh_dsgbsv(bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const double *A, bh_int lda, const double *B, bh_int ldb,
               double *X, bh_int ldx, void *work, size_t lwork, bh_int *ipiv32, bh_int *iters_host, bh_int *info_host) {
+    return dsgbsv_impl(n, kl, ku, nrhs, A, lda, B, ldb, X, ldx, work, lwork, nullptr, ipiv32, iters_host, info_host);
+}
+
+int bh_dsgbsv_noalloc(bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const double *A, bh_int lda, const double *B,
+                      bh_int ldb, double *X, bh_int ldx, bh_int *ipiv32, bh_int *iters_host, bh_int *info_host) {
+    BH_ARG(n >= 0, 1);
+    BH_ARG(kl >= 0, 2);
+    BH_ARG(ku >= 0, 3);
+    BH_ARG(nrhs >= 0, 4);
+    BH_ARG(lda >= kl + ku + 1, 6);
+    BH_ARG(ldb >= imax(1, n), 8);
+    BH_ARG(ldx >= imax(1, n), 10);
+    BH_ARG(iters_host != nullptr, 12);
+    BH_ARG(info_host != nullptr, 13);
+    const size_t need = std::max(mixed_layout(n, kl, ku, nrhs, nullptr).bytes, fallback_bytes(n, kl, ku));
+    if (need > kNoAllocBytes) {
+        set_error("bh_dsgbsv_noalloc: n=%lld kl=%lld ku=%lld nrhs=%lld needs %zu bytes of scratch; the static scratch "
+                  "holds %zu (use bh_dsgbsv with a caller workspace)",
+                  (long long)n, (long long)kl, (long long)ku, (long long)nrhs, need, kNoAllocBytes);
+        return BH_ERR_WORKSPACE;
+    }
+    int dev = -1;
+    BH_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices || g_noalloc_ptr[dev] == nullptr) {
+        const int le = (dev >= 0 && dev < kMaxDevices) ? g_noalloc_status[dev] : 0;
+        if (le)
+            set_error("bh_dsgbsv_noalloc: hipGetSymbolAddress of the static scratch failed on device %d: %s", dev,
+                      hipGetErrorString((hipError_t)le));
+        else
+            set_error("bh_dsgbsv_noalloc: no static scratch address for device %d (call bh_init(%d) first)", dev, dev);
+        return BH_ERR_NODEVICE;
+    }
+    void *p = g_noalloc_ptr[dev];
+    // the fallback overlays the refinement's workspace: by then it is dead
+    return dsgbsv_impl(n, kl, ku, nrhs, A, lda, B, ldb, X, ldx, p, kNoAllocBytes, p, ipiv32, iters_host, info_host);
+}
+
+}  // extern "C"
+
+namespace bh {
+
+static int dsgbsv_impl(i64 n, i64 kl, i64 ku, i64 nrhs, const double *A, i64 lda, const double *B, i64 ldb,
+                       double *X, i64 ldx, void *work, size_t lwork, void *fb, i64 *ipiv32, i64 *iters_host,
+                       i64 *info_host) {
     BH_ARG(n >= 0, 1);
     BH_ARG(kl >= 0, 2);
     BH_ARG(ku >= 0, 3);
@@ -235,12 +315,14 @@ int bh_dsgbsv(bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const double *A, bh_i
     double *ab64 = nullptr;
     i64 *ipiv64 = nullptr;
     {
-        const size_t fb = align256((size_t)ldab * n * sizeof(double));
-        void *sp = nullptr;
-        st = scratch_buffer(fb + (size_t)n * sizeof(i64), &sp);
-        if (st) return st;
+        const size_t fa = align256((size_t)ldab * n * sizeof(double));
+        void *sp = fb;
+        if (sp == nullptr) {
+            st = scratch_buffer(fallback_bytes(n, kl, ku), &sp);
+            if (st) return st;
+        }
         ab64 = (double *)sp;
-        ipiv64 = (i64 *)((char *)sp + fb);
+        ipiv64 = (i64 *)((char *)sp + fa);
     }
     st = bh_dgbpad(n, n, kl, ku, A, lda, ab64, ldab);
     i64 info64 = 0;
@@ -256,4 +338,4 @@ int bh_dsgbsv(bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const double *A, bh_i
     return st;
 }
 
-}  // extern "C"
+}  // namespace bh

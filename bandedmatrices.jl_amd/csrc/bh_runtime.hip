@@ -104,6 +104,9 @@ int bh_init(int device) {
         g_scratch_bytes = kScratchInitial;
     }
     g_device = device;
+    // the no-allocation entry point's scratch address on this device; a
+    // failure is recorded there and reported by that entry point only
+    (void)noalloc_setup(device);
     return BH_OK;
 }
 

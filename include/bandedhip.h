@@ -353,6 +353,19 @@ int bh_dsgbsv(bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const double *A, bh_i
               const double *B, bh_int ldb, double *X, bh_int ldx, void *work, size_t lwork,
               bh_int *ipiv32, bh_int *iters_host, bh_int *info_host);
 
+/* bh_dsgbsv without a workspace argument and without any device allocation:
+ * no reference counterpart; it is for callers that cannot allocate device
+ * memory (serving loops, harnesses that forbid hipMalloc).  The workspace and
+ * the FP64 fallback's factors live in one statically allocated 64 MiB device
+ * array per GPU (held from library load until process exit), so
+ * max(bh_dsgbsv_workspace bytes, 256-aligned (2kl+ku+1) n 8 + 8 n) must not
+ * exceed 64 MiB, else BH_ERR_WORKSPACE.  bh_init(device) must have run on the
+ * current device (else BH_ERR_NODEVICE).  Every call on a GPU shares that
+ * array: calls that overlap on one GPU (other host threads or streams) race. */
+int bh_dsgbsv_noalloc(bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const double *A, bh_int lda,
+                      const double *B, bh_int ldb, double *X, bh_int ldx, bh_int *ipiv32,
+                      bh_int *iters_host, bh_int *info_host);
+
 #ifdef __cplusplus
 }
 #endif
