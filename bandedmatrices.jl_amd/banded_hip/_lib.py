@@ -79,6 +79,9 @@ SIGNATURES["bh_batch_graph_destroy"] = (C.c_int, [vp])
 SIGNATURES["bh_dsgbpad"] = (C.c_int, [i64, i64, i64, i64, vp, i64, vp, i64])
 SIGNATURES["bh_dsgbsv_workspace"] = (C.c_int, [i64, i64, i64, i64, C.POINTER(C.c_size_t)])
 SIGNATURES["bh_dsgbsv"] = (C.c_int, [i64, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp, C.c_size_t, vp, pi64, pi64])
+SIGNATURES["bh_gbtrs_blocked_workspace"] = (C.c_int, [C.c_char, i64, i64, i64, C.POINTER(C.c_size_t)])
+for _p in "ds":
+    SIGNATURES[f"bh_{_p}gbtrs_blocked"] = (C.c_int, [i64, i64, i64, i64, vp, i64, vp, vp, i64, vp, C.c_size_t])
 SIGNATURES["bh_dsgbsv_noalloc"] = (C.c_int, [i64, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp, pi64, pi64])
 
 _lib = None

@@ -28,6 +28,16 @@ int scratch_buffer(size_t bytes, void **p);
 // Looks up, with `device` current, the device address of the statically
 // allocated scratch used by bh_dsgbsv_noalloc (once per device).
 int noalloc_setup(int device);
+// The blocked band solve (bh_gbtrs_mfma.hip): workspace bytes (0: not
+// applicable to these bands), the once-per-factorisation pre-pass, and the
+// forward + backward sweeps for nrhs columns of B.
+template <typename T>
+size_t trs_blocked_bytes(i64 n, i64 kl, i64 ku);
+template <typename T>
+int trs_blocked_prepare(i64 n, i64 kl, i64 ku, const T *ab, i64 ldab, const i64 *ipiv, void *work, hipStream_t st);
+template <typename T>
+int trs_blocked_solve(i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 ldab, const void *work, T *B, i64 ldb,
+                      hipStream_t st);
 void set_error(const char *fmt, ...);
 int hip_fail(hipError_t e, const char *what, const char *file, int line);
 

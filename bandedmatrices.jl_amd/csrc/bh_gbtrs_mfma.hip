@@ -1,0 +1,593 @@
+// bh_gbtrs_mfma.hip -- blocked band solve with the factors of bh_?gbtrf,
+// panels of PB rows applied on the matrix cores (gfx950 MFMA 16x16x4).
+//
+// The reference solves with LAPACK.gbtrs! (src/banded/linalg.jl:24-30):
+// xGBTRS applies, column by column, the row interchange of step j and the
+// rank-1 update of the rows below it (the forward sweep with L), then the
+// upper band solve with U (xTBSV).  Both sweeps are a chain of n dependent
+// steps.  Here the chain has n/PB links instead:
+//
+// Forward.  For panel P (rows j0 = P*PB .. j0+PB-1) the window is the RW =
+// kl+PB rows its interchanges and multipliers touch.  Applying the PB steps
+// of the panel to the window equals (with the panel's row permutation perm,
+// its multipliers in final row order L11 (unit lower, PB x PB) and L21
+// (kl x PB))
+//     Y1 = L11^{-1} (perm W)[0:PB]            the panel's finished rows
+//     W' = (perm W)[PB:RW] - L21 Y1           the rows the next panel sees
+// perm, L11^{-1} and L21 depend on the factors only; a parallel pre-pass
+// (one workgroup per panel) builds them once per factorisation.  L21 in
+// final order is xGBTRF's form of the multipliers before it "partially
+// undoes" the interchanges: each stored column c is moved to the positions
+// its rows reach after the panel's later interchanges.
+// Backward.  Panels from the last: X1 = U11^{-1} Y[j0:j0+PB]; then the kv =
+// kl+ku rows above the panel lose U[j0-kv:j0, j0:j0+PB] X1.
+//
+// One workgroup solves 16 right-hand sides; each link is two MFMA products
+// and two LDS-only barriers; the next panel's operands are prefetched into
+// registers during the current one.  Results equal xGBTRS's to rounding
+// (the panel's updates reach a row as one sum, the diagonal blocks through
+// their inverses), not bit for bit: this solve serves the mixed-precision
+// refinement, whose corrections need an accurate solve, not LAPACK's
+// operation order; bh_?gbtrs stays the bit-exact path.
+#include "bh_common.hpp"
+#include "bh_mfma.hpp"
+
+namespace bh {
+
+namespace {
+
+template <typename T>
+struct TStride {
+    static constexpr int value = sizeof(T) == 4 ? 20 : 18;  // LDS row stride of [rows][16] tiles
+};
+
+template <typename T, int PB>
+struct TrsLayout {
+    int n, kl, ku, kv, RW, KT, KTU, np;
+    // per-panel record offsets (elements of T) inside the pre-pass output
+    size_t linv_off, l21_off, uinv_off, per_panel, perm_ints;
+    static TrsLayout make(i64 n, i64 kl, i64 ku) {
+        TrsLayout L{};
+        L.n = (int)n;
+        L.kl = (int)kl;
+        L.ku = (int)ku;
+        L.kv = (int)(kl + ku);
+        L.RW = (int)kl + PB;
+        L.KT = (int)((kl + 15) / 16);
+        L.KTU = (int)((kl + ku + 15) / 16);
+        L.np = (int)((n + PB - 1) / PB);
+        L.linv_off = 0;
+        L.l21_off = (size_t)PB * PB;
+        L.uinv_off = L.l21_off + (size_t)L.KT * 16 * PB;
+        L.per_panel = L.uinv_off + (size_t)PB * PB;
+        L.perm_ints = (size_t)L.RW;
+        return L;
+    }
+};
+
+// ---------------------------------------------------------------- pre-pass
+// One workgroup per panel: perm, L11^{-1} and L21 in final row order, U11^{-1},
+// the matrices stored in MFMA A-operand order (16x4 blocks, lane i+16k holding
+// element (i, k)), so the solve kernels load them with one coalesced read each.
+template <typename T, int PB>
+__global__ __launch_bounds__(256) void trs_prep(const T *__restrict__ ab, i64 ldab, const i64 *__restrict__ ipiv,
+                                                TrsLayout<T, PB> L, int *__restrict__ permA, T *__restrict__ rec) {
+    extern __shared__ __align__(16) unsigned char tp_raw[];
+    constexpr int LS = PB + 1;
+    const int RW = L.RW, n = L.n, kl = L.kl, kv = L.kv;
+    T *Lf = reinterpret_cast<T *>(tp_raw);  // RW x LS
+    T *Ut = Lf + (size_t)RW * LS;           // PB x LS
+    T *Xi = Ut + PB * LS;                   // PB x LS
+    int *perm = reinterpret_cast<int *>(Xi + PB * LS);
+    int *F = perm + RW;
+    __shared__ int q[PB];
+    const int p = blockIdx.x, tid = threadIdx.x;
+    const int j0 = p * PB, nb = min(PB, n - j0);
+    for (int e = tid; e < RW * LS; e += 256) Lf[e] = T(0);
+    for (int r = tid; r < RW; r += 256) {
+        perm[r] = r;
+        F[r] = r;
+    }
+    if (tid < PB) {
+        int qq = tid;
+        if (tid < nb) {
+            const i64 piv = ipiv[j0 + tid] - 1 - j0;  // 0 <= piv - tid <= kl for valid factors
+            qq = (piv >= tid && piv < RW) ? (int)piv : tid;
+        }
+        q[tid] = qq;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        for (int c = 0; c < nb; ++c) {
+            const int a = perm[c];
+            perm[c] = perm[q[c]];
+            perm[q[c]] = a;
+        }
+    }
+    // multipliers of column c sit at positions c+1..c+kl after step c; the
+    // later interchanges of the panel carry position r to F_c(r)
+    for (int c = nb - 1; c >= 0; --c) {
+        const i64 col = (i64)(j0 + c) * ldab;
+        for (int r = c + 1 + tid; r <= c + kl && r < RW; r += 256)
+            if (j0 + r < n) Lf[F[r] * LS + c] = ab[col + (kv + r - c)];
+        __syncthreads();
+        if (tid == 0 && q[c] != c) {
+            const int a = F[c];
+            F[c] = F[q[c]];
+            F[q[c]] = a;
+        }
+        __syncthreads();
+    }
+    if (tid < PB) Lf[tid * LS + tid] = T(1);
+    // U11 (upper, identity past the last column)
+    for (int e = tid; e < PB * PB; e += 256) {
+        const int i = e / PB, j = e - i * PB;
+        T v = T(0);
+        if (j >= nb)
+            v = (i == j) ? T(1) : T(0);
+        else if (i <= j && j - i <= kv)
+            v = ab[(i64)(j0 + j) * ldab + (kv + i - j)];
+        Ut[i * LS + j] = v;
+    }
+    __syncthreads();
+    // L11^{-1}: column j by forward substitution (thread j)
+    if (tid < PB) {
+        const int j = tid;
+        for (int i = 0; i < PB; ++i) {
+            T x = (i == j) ? T(1) : T(0);
+            if (i > j) {
+                T s = T(0);
+                for (int k = j; k < i; ++k) s = dfma(Lf[i * LS + k], Xi[k * LS + j], s);
+                x = -s;
+            }
+            Xi[i * LS + j] = x;
+        }
+    }
+    __syncthreads();
+    T *out = rec + (size_t)p * L.per_panel;
+    for (int e = tid; e < PB * PB; e += 256) {
+        const int lane = e & 63, blk = e >> 6, s = blk % (PB / 4), ti = blk / (PB / 4);
+        out[L.linv_off + e] = Xi[(16 * ti + (lane & 15)) * LS + 4 * s + (lane >> 4)];
+    }
+    for (int e = tid; e < L.KT * 16 * PB; e += 256) {
+        const int lane = e & 63, blk = e >> 6, s = blk % (PB / 4), t = blk / (PB / 4);
+        const int r = PB + 16 * t + (lane & 15);
+        out[L.l21_off + e] = r < RW ? Lf[r * LS + 4 * s + (lane >> 4)] : T(0);
+    }
+    for (int r = tid; r < RW; r += 256) permA[(size_t)p * L.perm_ints + r] = perm[r];
+    __syncthreads();
+    // U11^{-1}: column j by back substitution (thread j)
+    if (tid < PB) {
+        const int j = tid;
+        for (int i = PB - 1; i >= 0; --i) {
+            T x = T(0);
+            if (i <= j) {
+                T s = (i == j) ? T(1) : T(0);
+                for (int k = i + 1; k <= j; ++k) s = dfma(-Ut[i * LS + k], Xi[k * LS + j], s);
+                x = s / Ut[i * LS + i];
+            }
+            Xi[i * LS + j] = x;
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < PB * PB; e += 256) {
+        const int lane = e & 63, blk = e >> 6, s = blk % (PB / 4), ti = blk / (PB / 4);
+        out[L.uinv_off + e] = Xi[(16 * ti + (lane & 15)) * LS + 4 * s + (lane >> 4)];
+    }
+}
+
+// Operands of one panel, loaded two panels ahead of their use (three sets
+// rotate), so the load latency is hidden behind two panels of work.
+template <typename T, int NS, int MT>
+struct PanelOps {
+    T a[NS];      // A operands of the diagonal-block product
+    T b[MT][NS];  // A operands of this wave's update tiles
+    T fresh;      // one element of the rows entering the window
+    int pn;       // one entry of the panel's permutation (forward)
+};
+
+// ---------------------------------------------------------------- forward sweep
+// B (n x nrhs, ldb) <- L^{-1} P B for the 16 columns starting at 16*blockIdx.x.
+template <typename T, int PB, int NW, int MT>
+__global__ __launch_bounds__(64 * NW) void trs_fwd(T *B, i64 ldb, int nrhs, TrsLayout<T, PB> L, const int *permA,
+                                                   const T *rec) {
+    using v4 = typename Mf<T>::v4;
+    constexpr int SW = TStride<T>::value;
+    constexpr int NS = PB / 4;          // K chunks of a panel product
+    constexpr int NP = PB / 16;         // 16-row tiles of a panel (MT: L21 tiles per wave)
+    using Ops = PanelOps<T, NS, MT>;
+    extern __shared__ __align__(16) unsigned char tf_raw[];
+    const int RW = L.RW, KT = L.KT, n = L.n, np = L.np;
+    const int RWP = 16 * ((RW + 15) / 16);
+    T *W = reinterpret_cast<T *>(tf_raw);  // [2][RWP][SW]
+    T *Yb = W + 2 * RWP * SW;              // [PB][SW]
+    int *pm = reinterpret_cast<int *>(Yb + PB * SW);  // [2][RW]
+    // w is wave-uniform: branches on it stay scalar.  The operand pointers are
+    // deliberately not __restrict__: that would let the compiler treat the
+    // prefetching loads as invariant and sink them to their uses.
+    const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int jc = lane & 15, kq = lane >> 4;
+    const int fr = tid >> 4, fj = tid & 15;  // PB*16 <= 64*NW threads carry one fresh element
+    const int c0 = 16 * blockIdx.x;
+    const bool colok = c0 + jc < nrhs;
+    // the 16 columns as a buffer: stores to rows past n (or columns past
+    // nrhs) go to an offset beyond it and are dropped without a branch
+    const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(
+        B + (i64)c0 * ldb, 0, (int)(16 * ldb * (i64)sizeof(T)), kQRsrcWord3);
+
+    // Every wave issues the same loads and stores in every panel (clamped
+    // indices, buffer stores), so the compiler's wait counts stay exact and
+    // a panel waits only for the loads of two panels before it.
+    auto issue = [&](int p, Ops &o) {
+        const T *rp = rec + (size_t)p * L.per_panel;
+        const int wa = min(w, NP - 1);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) o.a[s] = rp[L.linv_off + (size_t)(wa * NS + s) * 64 + lane];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const int t = min(w + m * NW, KT - 1);
+#pragma unroll
+            for (int s = 0; s < NS; ++s) o.b[m][s] = rp[L.l21_off + (size_t)(t * NS + s) * 64 + lane];
+        }
+        o.pn = permA[(size_t)p * L.perm_ints + min(tid, RW - 1)];
+        // loads from clamped (valid) addresses, then a select: no divergent branch
+        const int row = p * PB + RW + fr;
+        const bool fok = fr < PB && row < n && c0 + fj < nrhs;
+        const T fv = B[(i64)min(c0 + fj, nrhs - 1) * ldb + min(row, n - 1)];
+        o.fresh = fok ? fv : T(0);
+    };
+
+    // one panel: uses `u`, writes the next panel's permutation from `nx`,
+    // starts the loads of panel p+2 into `ld`
+    auto panel = [&](int p, const Ops &u, const Ops &nx, Ops &ld) {
+        const int cur = p & 1;
+        const int j0 = p * PB;
+        const T *Wc = W + cur * RWP * SW;
+        T *Wn = W + (cur ^ 1) * RWP * SW;
+        const int *pc = pm + cur * RW;
+        issue(min(p + 2, np - 1), ld);
+        // C operands of this wave's L21 tiles (read before Y1 exists)
+        T cin[MT][4];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const int t = w + m * NW;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = PB + 16 * t + drow<T>(kq, v);
+                const T cv = Wc[pc[min(r, RW - 1)] * SW + jc];
+                cin[m][v] = (t < KT && r < RW) ? cv : T(0);
+            }
+        }
+        // Y1 = L11^{-1} (perm W)[0:PB]
+        v4 acc = {T(0), T(0), T(0), T(0)};
+        if (w < NP) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) acc = Mf<T>::mma(u.a[s], Wc[pc[4 * s + kq] * SW + jc], acc);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Yb[(16 * w + drow<T>(kq, v)) * SW + jc] = acc[v];
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = 16 * w + drow<T>(kq, v);
+            const bool ok = w < NP && colok && j0 + r < n;
+            qbstore<T>(rB, ok ? (jc * (int)ldb + j0 + r) * (int)sizeof(T) : kQOob, acc[v]);
+        }
+        lds_barrier();
+        // W' = (perm W)[PB:RW] - L21 Y1
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const int t = w + m * NW;
+            if (t < KT) {
+                v4 acc = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+                for (int s = 0; s < NS; ++s) acc = Mf<T>::mma(u.b[m][s], Yb[(4 * s + kq) * SW + jc], acc);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = PB + 16 * t + drow<T>(kq, v);
+                    if (r < RW) Wn[(r - PB) * SW + jc] = cin[m][v] - acc[v];
+                }
+            }
+        }
+        if (fr < PB) Wn[(RW - PB + fr) * SW + fj] = u.fresh;
+        if (tid < RW) pm[(cur ^ 1) * RW + tid] = nx.pn;
+        lds_barrier();
+    };
+
+    Ops S0, S1, S2;
+    issue(0, S0);
+    issue(min(1, np - 1), S1);
+    // window of panel 0, and its permutation
+    for (int e = tid; e < RWP * 16; e += 64 * NW) {
+        const int r = e >> 4, j = e & 15;
+        W[r * SW + j] = (r < RW && r < n && c0 + j < nrhs) ? B[(i64)(c0 + j) * ldb + r] : T(0);
+    }
+    if (tid < RW) pm[tid] = S0.pn;
+    lds_barrier();
+    // panels past the last (at most two, to fill the rotation) store
+    // nothing: their rows are past n
+    for (int p = 0; p < np; p += 3) {
+        panel(p, S0, S1, S2);
+        panel(p + 1, S1, S2, S0);
+        panel(p + 2, S2, S0, S1);
+    }
+}
+
+// ---------------------------------------------------------------- backward sweep
+// B <- U^{-1} B (U: upper band of width kv = kl+ku in AB) for 16 columns.
+template <typename T, int PB, int NW, int MT>
+__global__ __launch_bounds__(64 * NW) void trs_bwd(T *B, i64 ldb, int nrhs, TrsLayout<T, PB> L, const T *ab, i64 ldab,
+                                                   const T *rec, int RB) {
+    using v4 = typename Mf<T>::v4;
+    constexpr int SW = TStride<T>::value;
+    constexpr int NS = PB / 4;
+    constexpr int NP = PB / 16;  // (MT: U tiles per wave)
+    using Ops = PanelOps<T, NS, MT>;
+    extern __shared__ __align__(16) unsigned char tb_raw[];
+    const int n = L.n, kv = L.kv, KTU = L.KTU, np = L.np;
+    const int top = 16 * KTU;  // rows above a panel that it updates (>= kv)
+    T *Yr = reinterpret_cast<T *>(tb_raw);  // ring [RB][SW], row i in slot i & (RB-1)
+    T *Xb = Yr + RB * SW;                   // [PB][SW]
+    const int msk = RB - 1;
+    // w is wave-uniform: branches on it stay scalar.  The operand pointers are
+    // deliberately not __restrict__: that would let the compiler treat the
+    // prefetching loads as invariant and sink them to their uses.
+    const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+    const int jc = lane & 15, kq = lane >> 4;
+    const int fr = tid >> 4, fj = tid & 15;
+    const int c0 = 16 * blockIdx.x;
+    const bool colok = c0 + jc < nrhs;
+    const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(
+        B + (i64)c0 * ldb, 0, (int)(16 * ldb * (i64)sizeof(T)), kQRsrcWord3);
+
+    // A operands: U11^{-1} tiles (waves < NP) and the U block above the
+    // panel; the rows [j0 - top - PB, j0 - top) that enter the ring during it
+    auto issue = [&](int p, Ops &o) {
+        const int j0 = p * PB;
+        {
+            const T *rp = rec + (size_t)p * L.per_panel + L.uinv_off;
+            const int wa = min(w, NP - 1);
+#pragma unroll
+            for (int s = 0; s < NS; ++s) o.a[s] = rp[(size_t)(wa * NS + s) * 64 + lane];
+        }
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const int t = w + m * NW;
+#pragma unroll
+            for (int s = 0; s < NS; ++s) {
+                const int i = j0 - top + 16 * t + jc;  // row of U
+                const int j = j0 + 4 * s + kq;         // column of U
+                const int d = kv + i - j;              // AB row; < kv since i < j0 <= j
+                const bool ok = t < KTU && i >= 0 && j < n && d >= 0;
+                const T v = ab[ok ? (i64)j * ldab + d : 0];
+                o.b[m][s] = ok ? v : T(0);
+            }
+        }
+        o.pn = 0;
+        const int frow = j0 - top - PB + fr;
+        const bool fok = fr < PB && frow >= 0 && c0 + fj < nrhs;
+        const T fv = B[(i64)min(c0 + fj, nrhs - 1) * ldb + max(frow, 0)];
+        o.fresh = fok ? fv : T(0);
+    };
+
+    // panel index q counts from the last panel (q = 0 is panel np-1); the
+    // same uniform load/store sequence as the forward sweep
+    auto panel = [&](int q, const Ops &u, Ops &ld) {
+        const int p = np - 1 - q;
+        const int j0 = p * PB;
+        issue(max(p - 2, 0), ld);
+        // X1 = U11^{-1} Y[j0:j0+PB]
+        v4 acc = {T(0), T(0), T(0), T(0)};
+        if (w < NP) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) acc = Mf<T>::mma(u.a[s], Yr[((j0 + 4 * s + kq) & msk) * SW + jc], acc);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Xb[(16 * w + drow<T>(kq, v)) * SW + jc] = acc[v];
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = 16 * w + drow<T>(kq, v);
+            const bool ok = w < NP && colok && j0 + r >= 0 && j0 + r < n;
+            qbstore<T>(rB, ok ? (jc * (int)ldb + j0 + r) * (int)sizeof(T) : kQOob, acc[v]);
+        }
+        lds_barrier();
+        // Y[j0-top : j0] -= U[j0-top : j0, j0 : j0+PB] X1
+#pragma unroll
+        for (int m = 0; m < MT; ++m) {
+            const int t = w + m * NW;
+            if (t < KTU) {
+                v4 acc = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+                for (int s = 0; s < NS; ++s) acc = Mf<T>::mma(u.b[m][s], Xb[(4 * s + kq) * SW + jc], acc);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int row = j0 - top + 16 * t + drow<T>(kq, v);
+                    if (row >= 0) Yr[(row & msk) * SW + jc] -= acc[v];
+                }
+            }
+        }
+        const int frow = j0 - top - PB + fr;
+        if (fr < PB && frow >= 0) Yr[(frow & msk) * SW + fj] = u.fresh;
+        lds_barrier();
+    };
+
+    Ops S0, S1, S2;
+    issue(np - 1, S0);
+    issue(max(np - 2, 0), S1);
+    // ring: rows [jl - top, jl + PB) of the last panel (zero past n)
+    const int jl = (np - 1) * PB;
+    for (int e = tid; e < (top + PB) * 16; e += 64 * NW) {
+        const int rr = e >> 4, j = e & 15;
+        const int row = jl - top + rr;
+        T v = T(0);
+        if (row >= 0 && row < n && c0 + j < nrhs) v = B[(i64)(c0 + j) * ldb + row];
+        Yr[(row & msk) * SW + j] = v;
+    }
+    lds_barrier();
+    // panels before the first (at most two) touch only negative rows: their
+    // stores are dropped and their ring updates are skipped
+    for (int q = 0; q < np; q += 3) {
+        panel(q, S0, S2);
+        panel(q + 1, S1, S0);
+        panel(q + 2, S2, S1);
+    }
+}
+
+constexpr int kTrsPB = 32;
+constexpr int kTrsNW = 8;
+
+}  // namespace
+
+// Bytes of the pre-pass output for (n, kl, ku): 0 when the blocked solve does
+// not apply (then the caller uses bh_?gbtrs).
+template <typename T>
+size_t trs_blocked_bytes(i64 n, i64 kl, i64 ku) {
+    using Lt = TrsLayout<T, kTrsPB>;
+    constexpr int SW = TStride<T>::value;
+    // 16 columns of B (leading dimension >= n) must fit one 31-bit buffer offset range
+    if (n <= 0 || kl < 1 || 16 * n * (i64)sizeof(T) >= kQOob) return 0;
+    // update tiles per wave: at most 4 (FP32); FP64 registers hold 2 forward, 3 backward
+    const i64 mtf = sizeof(T) == 4 ? 4 : 2, mtb = sizeof(T) == 4 ? 4 : 3;
+    if (kl > 16 * mtf * kTrsNW || kl + ku > 16 * mtb * kTrsNW) return 0;
+    const Lt L = Lt::make(n, kl, ku);
+    if (L.RW > 64 * kTrsNW) return 0;  // one thread per window row carries the next permutation
+    const int RWP = 16 * ((L.RW + 15) / 16);
+    const size_t fwd_lds = (size_t)(2 * RWP + kTrsPB) * SW * sizeof(T) + 2 * L.RW * sizeof(int);
+    int RB = 1;
+    while (RB < 16 * L.KTU + 2 * kTrsPB) RB <<= 1;
+    const size_t bwd_lds = (size_t)(RB + kTrsPB) * SW * sizeof(T);
+    const size_t prep_lds = ((size_t)L.RW * (kTrsPB + 1) + 2 * kTrsPB * (kTrsPB + 1)) * sizeof(T) + 2 * L.RW * sizeof(int);
+    if (fwd_lds > 160 * 1024 || bwd_lds > 160 * 1024 || prep_lds > 160 * 1024) return 0;
+    const size_t recs = (size_t)L.np * L.per_panel * sizeof(T);
+    const size_t perms = (size_t)L.np * L.perm_ints * sizeof(int);
+    return ((recs + 255) & ~(size_t)255) + ((perms + 255) & ~(size_t)255);
+}
+
+template <typename T>
+int trs_blocked_prepare(i64 n, i64 kl, i64 ku, const T *ab, i64 ldab, const i64 *ipiv, void *work, hipStream_t st) {
+    using Lt = TrsLayout<T, kTrsPB>;
+    const Lt L = Lt::make(n, kl, ku);
+    const size_t recs = (size_t)L.np * L.per_panel * sizeof(T);
+    T *rec = (T *)work;
+    int *perm = (int *)((char *)work + ((recs + 255) & ~(size_t)255));
+    const size_t lds = ((size_t)L.RW * (kTrsPB + 1) + 2 * kTrsPB * (kTrsPB + 1)) * sizeof(T) + 2 * L.RW * sizeof(int);
+    auto kern = trs_prep<T, kTrsPB>;
+    if (lds > 64 * 1024)
+        BH_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(kern, dim3((unsigned)L.np), dim3(256), lds, st, ab, ldab, ipiv, L, perm, rec);
+    BH_LAUNCH_CHECK();
+    return BH_OK;
+}
+
+template <typename T>
+int trs_blocked_solve(i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 ldab, const void *work, T *B, i64 ldb,
+                      hipStream_t st) {
+    using Lt = TrsLayout<T, kTrsPB>;
+    constexpr int SW = TStride<T>::value;
+    if (nrhs <= 0 || n <= 0) return BH_OK;
+    if (16 * ldb * (i64)sizeof(T) >= kQOob) {
+        set_error("gbtrs_blocked: ldb = %lld is too large for 16 columns in one buffer range", (long long)ldb);
+        return BH_ERR_WORKSPACE;
+    }
+    const Lt L = Lt::make(n, kl, ku);
+    const size_t recs = (size_t)L.np * L.per_panel * sizeof(T);
+    const T *rec = (const T *)work;
+    const int *perm = (const int *)((const char *)work + ((recs + 255) & ~(size_t)255));
+    const dim3 grid((unsigned)((nrhs + 15) / 16));
+    const int RWP = 16 * ((L.RW + 15) / 16);
+    const size_t fl = (size_t)(2 * RWP + kTrsPB) * SW * sizeof(T) + 2 * L.RW * sizeof(int);
+    const int mtf = (L.KT + kTrsNW - 1) / kTrsNW;
+    void (*kf)(T *, i64, int, Lt, const int *, const T *) = nullptr;
+    if constexpr (sizeof(T) == 4) {
+        kf = mtf <= 1 ? trs_fwd<T, kTrsPB, kTrsNW, 1>
+           : mtf == 2 ? trs_fwd<T, kTrsPB, kTrsNW, 2>
+           : mtf == 3 ? trs_fwd<T, kTrsPB, kTrsNW, 3>
+                      : trs_fwd<T, kTrsPB, kTrsNW, 4>;
+    } else {
+        if (mtf > 2) return BH_ERR_WORKSPACE;
+        kf = mtf <= 1 ? trs_fwd<T, kTrsPB, kTrsNW, 1> : trs_fwd<T, kTrsPB, kTrsNW, 2>;
+    }
+    if (fl > 64 * 1024) BH_HIP(hipFuncSetAttribute((const void *)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)fl));
+    hipLaunchKernelGGL(kf, grid, dim3(64 * kTrsNW), fl, st, B, ldb, (int)nrhs, L, perm, rec);
+    BH_LAUNCH_CHECK();
+    int RB = 1;
+    while (RB < 16 * L.KTU + 2 * kTrsPB) RB <<= 1;
+    const size_t bl = (size_t)(RB + kTrsPB) * SW * sizeof(T);
+    const int mtb = (L.KTU + kTrsNW - 1) / kTrsNW;
+    void (*kb)(T *, i64, int, Lt, const T *, i64, const T *, int) = nullptr;
+    if constexpr (sizeof(T) == 4) {
+        kb = mtb <= 1 ? trs_bwd<T, kTrsPB, kTrsNW, 1>
+           : mtb == 2 ? trs_bwd<T, kTrsPB, kTrsNW, 2>
+           : mtb == 3 ? trs_bwd<T, kTrsPB, kTrsNW, 3>
+                      : trs_bwd<T, kTrsPB, kTrsNW, 4>;
+    } else {
+        if (mtb > 3) return BH_ERR_WORKSPACE;
+        kb = mtb <= 1 ? trs_bwd<T, kTrsPB, kTrsNW, 1>
+           : mtb == 2 ? trs_bwd<T, kTrsPB, kTrsNW, 2>
+                      : trs_bwd<T, kTrsPB, kTrsNW, 3>;
+    }
+    if (bl > 64 * 1024) BH_HIP(hipFuncSetAttribute((const void *)kb, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bl));
+    hipLaunchKernelGGL(kb, grid, dim3(64 * kTrsNW), bl, st, B, ldb, (int)nrhs, L, ab, ldab, rec, RB);
+    BH_LAUNCH_CHECK();
+    return BH_OK;
+}
+
+template size_t trs_blocked_bytes<float>(i64, i64, i64);
+template size_t trs_blocked_bytes<double>(i64, i64, i64);
+template int trs_blocked_prepare<float>(i64, i64, i64, const float *, i64, const i64 *, void *, hipStream_t);
+template int trs_blocked_prepare<double>(i64, i64, i64, const double *, i64, const i64 *, void *, hipStream_t);
+template int trs_blocked_solve<float>(i64, i64, i64, i64, const float *, i64, const void *, float *, i64, hipStream_t);
+template int trs_blocked_solve<double>(i64, i64, i64, i64, const double *, i64, const void *, double *, i64,
+                                       hipStream_t);
+
+}  // namespace bh
+
+using namespace bh;
+
+template <typename T>
+static int gbtrs_blocked_impl(i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 ldab, const i64 *ipiv, T *B, i64 ldb,
+                              void *work, size_t lwork) {
+    BH_ARG(n >= 0, 1);
+    BH_ARG(kl >= 0, 2);
+    BH_ARG(ku >= 0, 3);
+    BH_ARG(nrhs >= 0, 4);
+    BH_ARG(ldab >= 2 * kl + ku + 1, 6);
+    BH_ARG(ldb >= imax(1, n), 9);
+    if (n == 0 || nrhs == 0) return BH_OK;
+    const size_t need = trs_blocked_bytes<T>(n, kl, ku);
+    if (need == 0) {
+        set_error("gbtrs_blocked: bands (%lld,%lld) are outside the blocked solve's range (FP32: 1 <= kl <= 480, "
+                  "kl+ku <= 512; FP64: 1 <= kl <= 256, kl+ku <= 384); use bh_?gbtrs", (long long)kl, (long long)ku);
+        return BH_ERR_WORKSPACE;
+    }
+    if (work == nullptr || lwork < need) {
+        set_error("gbtrs_blocked: workspace of %zu bytes needed, %zu given", need, lwork);
+        return BH_ERR_WORKSPACE;
+    }
+    int st = trs_blocked_prepare<T>(n, kl, ku, ab, ldab, ipiv, work, stream());
+    if (st) return st;
+    return trs_blocked_solve<T>(n, kl, ku, nrhs, ab, ldab, work, B, ldb, stream());
+}
+
+extern "C" {
+
+int bh_gbtrs_blocked_workspace(char prec, bh_int n, bh_int kl, bh_int ku, size_t *bytes) {
+    BH_ARG(prec == 'd' || prec == 's', 1);
+    BH_ARG(n >= 0, 2);
+    BH_ARG(kl >= 0, 3);
+    BH_ARG(ku >= 0, 4);
+    BH_ARG(bytes != nullptr, 5);
+    *bytes = prec == 'd' ? trs_blocked_bytes<double>(n, kl, ku) : trs_blocked_bytes<float>(n, kl, ku);
+    return BH_OK;
+}
+
+int bh_dgbtrs_blocked(bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const double *AB, bh_int ldab, const bh_int *ipiv,
+                      double *B, bh_int ldb, void *work, size_t lwork) {
+    return gbtrs_blocked_impl<double>(n, kl, ku, nrhs, AB, ldab, ipiv, B, ldb, work, lwork);
+}
+
+int bh_sgbtrs_blocked(bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const float *AB, bh_int ldab, const bh_int *ipiv,
+                      float *B, bh_int ldb, void *work, size_t lwork) {
+    return gbtrs_blocked_impl<float>(n, kl, ku, nrhs, AB, ldab, ipiv, B, ldb, work, lwork);
+}
+
+}  // extern "C"

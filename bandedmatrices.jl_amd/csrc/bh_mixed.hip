@@ -16,6 +16,9 @@
 // Residuals, norms and the solution are FP64 throughout.
 #include <math.h>
 
+#include <stdlib.h>
+#include <string.h>
+
 #include <algorithm>
 #include <vector>
 
@@ -84,7 +87,19 @@ __global__ void fill_zero_kernel(double *p, i64 cnt) {
 
 static inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// The FP32 solves of the refinement use the blocked MFMA solve (its
+// operands built once per factorisation) where the bands allow it, else
+// bh_sgbtrs.  BANDEDHIP_MIXED_TRS=ring|blocked overrides the choice.
+static bool mixed_blocked_trs(i64 n, i64 kl, i64 ku) {
+    if (trs_blocked_bytes<float>(n, kl, ku) == 0) return false;
+    const char *e = getenv("BANDEDHIP_MIXED_TRS");
+    if (e && !strcmp(e, "ring")) return false;
+    if (e && !strcmp(e, "blocked")) return true;
+    return kl >= 16;
+}
+
 struct MixedWs {
+    void *trs;  // blocked-solve operands (nullptr: bh_sgbtrs)
     float *ab32;
     i64 *ipiv;
     i64 *info;
@@ -111,6 +126,11 @@ static MixedWs mixed_layout(i64 n, i64 kl, i64 ku, i64 nrhs, void *base) {
     off += align256((size_t)n * nrhs * sizeof(float));
     w.norms = (double *)(p + off);
     off += align256((size_t)(1 + 2 * nrhs) * sizeof(double));
+    w.trs = nullptr;
+    if (mixed_blocked_trs(n, kl, ku)) {
+        w.trs = p + off;
+        off += align256(trs_blocked_bytes<float>(n, kl, ku));
+    }
     w.bytes = off;
     return w;
 }
@@ -259,7 +279,12 @@ static int dsgbsv_impl(i64 n, i64 kl, i64 ku, i64 nrhs, const double *A, i64 lda
         // x0 = solve in FP32
         hipLaunchKernelGGL(d2s_kernel, grid1(n * nrhs), dim3(256), 0, s, n, nrhs, B, ldb, w.r32, n);
         BH_LAUNCH_CHECK();
-        st = bh_sgbtrs('N', n, kl, ku, nrhs, w.ab32, ldab, w.ipiv, w.r32, n);
+        if (w.trs) {
+            st = trs_blocked_prepare<float>(n, kl, ku, w.ab32, ldab, w.ipiv, w.trs, s);
+            if (!st) st = trs_blocked_solve<float>(n, kl, ku, nrhs, w.ab32, ldab, w.trs, w.r32, n, s);
+        } else {
+            st = bh_sgbtrs('N', n, kl, ku, nrhs, w.ab32, ldab, w.ipiv, w.r32, n);
+        }
         if (st) return st;
         hipLaunchKernelGGL(s2d_add_kernel, grid1(n * nrhs), dim3(256), 0, s, n, nrhs, w.r32, n, X, ldx, 1);
         BH_LAUNCH_CHECK();
@@ -300,7 +325,8 @@ static int dsgbsv_impl(i64 n, i64 kl, i64 ku, i64 nrhs, const double *A, i64 lda
             // d = A32 \ r ; x += d
             hipLaunchKernelGGL(d2s_kernel, grid1(n * nrhs), dim3(256), 0, s, n, nrhs, w.r, n, w.r32, n);
             BH_LAUNCH_CHECK();
-            st = bh_sgbtrs('N', n, kl, ku, nrhs, w.ab32, ldab, w.ipiv, w.r32, n);
+            st = w.trs ? trs_blocked_solve<float>(n, kl, ku, nrhs, w.ab32, ldab, w.trs, w.r32, n, s)
+                       : bh_sgbtrs('N', n, kl, ku, nrhs, w.ab32, ldab, w.ipiv, w.r32, n);
             if (st) return st;
             hipLaunchKernelGGL(s2d_add_kernel, grid1(n * nrhs), dim3(256), 0, s, n, nrhs, w.r32, n, X, ldx, 0);
             BH_LAUNCH_CHECK();

@@ -137,6 +137,21 @@ int bh_dgbtrs(char trans, bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const dou
 int bh_sgbtrs(char trans, bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const float *AB,
               bh_int ldab, const bh_int *ipiv, float *B, bh_int ldb);
 
+/* Blocked solve A X = B ('N') with the factors of bh_?gbtrf: panels of 32
+ * rows on the matrix cores, 16 right-hand sides per workgroup.  Same result
+ * as bh_?gbtrs to rounding (the diagonal blocks through their inverses, a
+ * panel's updates summed at once), not bit for bit; for many right-hand
+ * sides with wide bands.  work: device workspace of
+ * bh_gbtrs_blocked_workspace(prec 'd'/'s', ...) bytes (0: bands outside the
+ * supported range -- FP32 1 <= kl <= 480, kl+ku <= 512; FP64 1 <= kl <= 256,
+ * kl+ku <= 384 -- then BH_ERR_WORKSPACE).
+ * Same job as LAPACK.gbtrs! in ldiv!(::BandedLU, B) (src/banded/linalg.jl:24-30). */
+int bh_gbtrs_blocked_workspace(char prec, bh_int n, bh_int kl, bh_int ku, size_t *bytes);
+int bh_dgbtrs_blocked(bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const double *AB, bh_int ldab,
+                      const bh_int *ipiv, double *B, bh_int ldb, void *work, size_t lwork);
+int bh_sgbtrs_blocked(bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const float *AB, bh_int ldab,
+                      const bh_int *ipiv, float *B, bh_int ldb, void *work, size_t lwork);
+
 /* ------------------------------------------------------------- banded QR */
 /* Householder QR in band storage: D is the padded copy (bh_?gbpad with
  * kl = l, ku = u_orig) viewed as bands (l, u) with u = l + u_orig, ldd >= l+u+1,
