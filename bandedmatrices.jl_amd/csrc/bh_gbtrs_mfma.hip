@@ -177,14 +177,22 @@ __global__ __launch_bounds__(256) void trs_prep(const T *__restrict__ ab, i64 ld
 }
 
 // Operands of one panel, loaded two panels ahead of their use (three sets
-// rotate), so the load latency is hidden behind two panels of work.
+// rotate), so the load latency is hidden behind two panels of work.  Every
+// wave issues the same buffer loads and stores in every panel (clamped
+// indices; out-of-range offsets read 0 and drop stores), so the compiler's
+// wait counts stay exact: a panel waits only for loads two panels old.
 template <typename T, int NS, int MT>
 struct PanelOps {
-    T a[NS];      // A operands of the diagonal-block product
-    T b[MT][NS];  // A operands of this wave's update tiles
-    T fresh;      // one element of the rows entering the window
-    int pn;       // one entry of the panel's permutation (forward)
+    T a[NS];       // A operands of the diagonal-block product
+    T b[MT][NS];   // A operands of this wave's update tiles
+    int g[NS];     // forward: window rows gathered as B operands (rows 4s+kq)
+    int c[MT][4];  // forward: window rows of the update tiles' C operands
+    T fresh;       // one element of the rows entering the window
 };
+
+__device__ __forceinline__ int bload_i32(__amdgpu_buffer_rsrc_t r, int off) {
+    return (int)__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+}
 
 // ---------------------------------------------------------------- forward sweep
 // B (n x nrhs, ldb) <- L^{-1} P B for the 16 columns starting at 16*blockIdx.x.
@@ -193,122 +201,127 @@ __global__ __launch_bounds__(64 * NW) void trs_fwd(T *B, i64 ldb, int nrhs, TrsL
                                                    const T *rec) {
     using v4 = typename Mf<T>::v4;
     constexpr int SW = TStride<T>::value;
-    constexpr int NS = PB / 4;          // K chunks of a panel product
-    constexpr int NP = PB / 16;         // 16-row tiles of a panel (MT: L21 tiles per wave)
+    constexpr int NS = PB / 4;   // K chunks of a panel product
+    constexpr int NP = PB / 16;  // 16-row tiles of a panel (MT: L21 tiles per wave)
+    constexpr int ES = (int)sizeof(T);
     using Ops = PanelOps<T, NS, MT>;
     extern __shared__ __align__(16) unsigned char tf_raw[];
     const int RW = L.RW, KT = L.KT, n = L.n, np = L.np;
     const int RWP = 16 * ((RW + 15) / 16);
     T *W = reinterpret_cast<T *>(tf_raw);  // [2][RWP][SW]
     T *Yb = W + 2 * RWP * SW;              // [PB][SW]
-    int *pm = reinterpret_cast<int *>(Yb + PB * SW);  // [2][RW]
-    // w is wave-uniform: branches on it stay scalar.  The operand pointers are
-    // deliberately not __restrict__: that would let the compiler treat the
-    // prefetching loads as invariant and sink them to their uses.
+    // w is wave-uniform (branches on it stay scalar)
     const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int jc = lane & 15, kq = lane >> 4;
     const int fr = tid >> 4, fj = tid & 15;  // PB*16 <= 64*NW threads carry one fresh element
     const int c0 = 16 * blockIdx.x;
     const bool colok = c0 + jc < nrhs;
-    // the 16 columns as a buffer: stores to rows past n (or columns past
-    // nrhs) go to an offset beyond it and are dropped without a branch
-    const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(
-        B + (i64)c0 * ldb, 0, (int)(16 * ldb * (i64)sizeof(T)), kQRsrcWord3);
+    const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(B + (i64)c0 * ldb, 0, (int)(16 * ldb * ES),
+                                                                        kQRsrcWord3);
+    // per-lane byte offsets inside a panel record and its permutation
+    const int offA = (int)((L.linv_off + (size_t)min(w, NP - 1) * NS * 64 + lane) * ES);
+    int offB[MT], rowC[MT][4];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+        const int t = min(w + m * NW, KT - 1);
+        offB[m] = (int)((L.l21_off + (size_t)t * NS * 64 + lane) * ES);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) rowC[m][v] = min(PB + 16 * t + drow<T>(kq, v), RW - 1);
+    }
 
-    // Every wave issues the same loads and stores in every panel (clamped
-    // indices, buffer stores), so the compiler's wait counts stay exact and
-    // a panel waits only for the loads of two panels before it.
     auto issue = [&](int p, Ops &o) {
-        const T *rp = rec + (size_t)p * L.per_panel;
-        const int wa = min(w, NP - 1);
+        const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<T *>(rec) + (size_t)p * L.per_panel, 0, (int)(L.per_panel * ES), kQRsrcWord3);
+        const __amdgpu_buffer_rsrc_t rp = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<int *>(permA) + (size_t)p * L.perm_ints, 0, RW * 4, kQRsrcWord3);
 #pragma unroll
-        for (int s = 0; s < NS; ++s) o.a[s] = rp[L.linv_off + (size_t)(wa * NS + s) * 64 + lane];
+        for (int s = 0; s < NS; ++s) o.a[s] = qbload<T>(rr, offA + s * 64 * ES);
 #pragma unroll
-        for (int m = 0; m < MT; ++m) {
-            const int t = min(w + m * NW, KT - 1);
+        for (int m = 0; m < MT; ++m)
 #pragma unroll
-            for (int s = 0; s < NS; ++s) o.b[m][s] = rp[L.l21_off + (size_t)(t * NS + s) * 64 + lane];
-        }
-        o.pn = permA[(size_t)p * L.perm_ints + min(tid, RW - 1)];
-        // loads from clamped (valid) addresses, then a select: no divergent branch
+            for (int s = 0; s < NS; ++s) o.b[m][s] = qbload<T>(rr, offB[m] + s * 64 * ES);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) o.g[s] = bload_i32(rp, (4 * s + kq) * 4);
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) o.c[m][v] = bload_i32(rp, rowC[m][v] * 4);
         const int row = p * PB + RW + fr;
-        const bool fok = fr < PB && row < n && c0 + fj < nrhs;
-        const T fv = B[(i64)min(c0 + fj, nrhs - 1) * ldb + min(row, n - 1)];
-        o.fresh = fok ? fv : T(0);
+        o.fresh = qbload<T>(rB, (fr < PB && row < n && c0 + fj < nrhs) ? (fj * (int)ldb + row) * ES : kQOob);
     };
 
-    // one panel: uses `u`, writes the next panel's permutation from `nx`,
-    // starts the loads of panel p+2 into `ld`
-    auto panel = [&](int p, const Ops &u, const Ops &nx, Ops &ld) {
+    // one panel: uses `u`, starts the loads of panel p+2 into `ld`
+    auto panel = [&](int p, const Ops &u, Ops &ld) {
         const int cur = p & 1;
         const int j0 = p * PB;
         const T *Wc = W + cur * RWP * SW;
         T *Wn = W + (cur ^ 1) * RWP * SW;
-        const int *pc = pm + cur * RW;
-        issue(min(p + 2, np - 1), ld);
-        // C operands of this wave's L21 tiles (read before Y1 exists)
+        // Y1 = L11^{-1} (perm W)[0:PB], two interleaved accumulations
+        v4 y = {T(0), T(0), T(0), T(0)};
+        if (w < NP) {
+            v4 y1 = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+            for (int s = 0; s < NS; s += 2) {
+                y = Mf<T>::mma(u.a[s], Wc[u.g[s] * SW + jc], y);
+                y1 = Mf<T>::mma(u.a[s + 1], Wc[u.g[s + 1] * SW + jc], y1);
+            }
+            y += y1;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Yb[(16 * w + drow<T>(kq, v)) * SW + jc] = y[v];
+        }
+        // C operands of the update tiles (independent of Y1)
         T cin[MT][4];
 #pragma unroll
-        for (int m = 0; m < MT; ++m) {
-            const int t = w + m * NW;
+        for (int m = 0; m < MT; ++m)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int r = PB + 16 * t + drow<T>(kq, v);
-                const T cv = Wc[pc[min(r, RW - 1)] * SW + jc];
-                cin[m][v] = (t < KT && r < RW) ? cv : T(0);
-            }
-        }
-        // Y1 = L11^{-1} (perm W)[0:PB]
-        v4 acc = {T(0), T(0), T(0), T(0)};
-        if (w < NP) {
-#pragma unroll
-            for (int s = 0; s < NS; ++s) acc = Mf<T>::mma(u.a[s], Wc[pc[4 * s + kq] * SW + jc], acc);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) Yb[(16 * w + drow<T>(kq, v)) * SW + jc] = acc[v];
-        }
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int r = 16 * w + drow<T>(kq, v);
-            const bool ok = w < NP && colok && j0 + r < n;
-            qbstore<T>(rB, ok ? (jc * (int)ldb + j0 + r) * (int)sizeof(T) : kQOob, acc[v]);
-        }
+            for (int v = 0; v < 4; ++v) cin[m][v] = Wc[u.c[m][v] * SW + jc];
         lds_barrier();
+        issue(min(p + 2, np - 1), ld);
         // W' = (perm W)[PB:RW] - L21 Y1
 #pragma unroll
         for (int m = 0; m < MT; ++m) {
             const int t = w + m * NW;
             if (t < KT) {
-                v4 acc = {T(0), T(0), T(0), T(0)};
+                v4 z = {T(0), T(0), T(0), T(0)}, z1 = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-                for (int s = 0; s < NS; ++s) acc = Mf<T>::mma(u.b[m][s], Yb[(4 * s + kq) * SW + jc], acc);
+                for (int s = 0; s < NS; s += 2) {
+                    z = Mf<T>::mma(u.b[m][s], Yb[(4 * s + kq) * SW + jc], z);
+                    z1 = Mf<T>::mma(u.b[m][s + 1], Yb[(4 * s + 4 + kq) * SW + jc], z1);
+                }
+                z += z1;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int r = PB + 16 * t + drow<T>(kq, v);
-                    if (r < RW) Wn[(r - PB) * SW + jc] = cin[m][v] - acc[v];
+                    if (r < RW) Wn[(r - PB) * SW + jc] = cin[m][v] - z[v];
                 }
             }
         }
         if (fr < PB) Wn[(RW - PB + fr) * SW + fj] = u.fresh;
-        if (tid < RW) pm[(cur ^ 1) * RW + tid] = nx.pn;
+        // Y1 to B (rows past n, columns past nrhs and waves past NP: dropped)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = 16 * w + drow<T>(kq, v);
+            const bool ok = w < NP && colok && j0 + r < n;
+            qbstore<T>(rB, ok ? (jc * (int)ldb + j0 + r) * ES : kQOob, y[v]);
+        }
         lds_barrier();
     };
 
     Ops S0, S1, S2;
     issue(0, S0);
     issue(min(1, np - 1), S1);
-    // window of panel 0, and its permutation
+    // window of panel 0
     for (int e = tid; e < RWP * 16; e += 64 * NW) {
         const int r = e >> 4, j = e & 15;
-        W[r * SW + j] = (r < RW && r < n && c0 + j < nrhs) ? B[(i64)(c0 + j) * ldb + r] : T(0);
+        W[r * SW + j] = qbload<T>(rB, (r < RW && r < n && c0 + j < nrhs) ? (j * (int)ldb + r) * ES : kQOob);
     }
-    if (tid < RW) pm[tid] = S0.pn;
     lds_barrier();
     // panels past the last (at most two, to fill the rotation) store
     // nothing: their rows are past n
     for (int p = 0; p < np; p += 3) {
-        panel(p, S0, S1, S2);
-        panel(p + 1, S1, S2, S0);
-        panel(p + 2, S2, S0, S1);
+        panel(p, S0, S2);
+        panel(p + 1, S1, S0);
+        panel(p + 2, S2, S1);
     }
 }
 
@@ -321,6 +334,7 @@ __global__ __launch_bounds__(64 * NW) void trs_bwd(T *B, i64 ldb, int nrhs, TrsL
     constexpr int SW = TStride<T>::value;
     constexpr int NS = PB / 4;
     constexpr int NP = PB / 16;  // (MT: U tiles per wave)
+    constexpr int ES = (int)sizeof(T);
     using Ops = PanelOps<T, NS, MT>;
     extern __shared__ __align__(16) unsigned char tb_raw[];
     const int n = L.n, kv = L.kv, KTU = L.KTU, np = L.np;
@@ -328,85 +342,93 @@ __global__ __launch_bounds__(64 * NW) void trs_bwd(T *B, i64 ldb, int nrhs, TrsL
     T *Yr = reinterpret_cast<T *>(tb_raw);  // ring [RB][SW], row i in slot i & (RB-1)
     T *Xb = Yr + RB * SW;                   // [PB][SW]
     const int msk = RB - 1;
-    // w is wave-uniform: branches on it stay scalar.  The operand pointers are
-    // deliberately not __restrict__: that would let the compiler treat the
-    // prefetching loads as invariant and sink them to their uses.
     const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int jc = lane & 15, kq = lane >> 4;
     const int fr = tid >> 4, fj = tid & 15;
     const int c0 = 16 * blockIdx.x;
     const bool colok = c0 + jc < nrhs;
-    const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(
-        B + (i64)c0 * ldb, 0, (int)(16 * ldb * (i64)sizeof(T)), kQRsrcWord3);
+    const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(B + (i64)c0 * ldb, 0, (int)(16 * ldb * ES),
+                                                                        kQRsrcWord3);
+    const int offA = (int)((L.uinv_off + (size_t)min(w, NP - 1) * NS * 64 + lane) * ES);
+    // U(i, j) for i = j0 - top + 16t + jc, j = j0 + 4s + kq sits at element
+    // (4s+kq) ldab + d of the panel's columns, d = kv + i - j the AB row;
+    // d < 0 is outside the band (offset sent out of range: reads 0).  Rows
+    // i < 0 read corner slots whose products land in rows that are skipped.
+    const int li = (int)ldab;
+    int offU[MT][NS];
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+        const int t = min(w + m * NW, KTU - 1);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+            const int d = kv - top + 16 * t + jc - 4 * s - kq;
+            offU[m][s] = d >= 0 ? ((4 * s + kq) * li + d) * ES : kQOob;
+        }
+    }
 
-    // A operands: U11^{-1} tiles (waves < NP) and the U block above the
-    // panel; the rows [j0 - top - PB, j0 - top) that enter the ring during it
     auto issue = [&](int p, Ops &o) {
         const int j0 = p * PB;
-        {
-            const T *rp = rec + (size_t)p * L.per_panel + L.uinv_off;
-            const int wa = min(w, NP - 1);
+        const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<T *>(rec) + (size_t)p * L.per_panel, 0, (int)(L.per_panel * ES), kQRsrcWord3);
+        // the panel's columns, up to column n (columns past it read 0)
+        const __amdgpu_buffer_rsrc_t ru = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<T *>(ab) + (i64)j0 * ldab, 0, (int)((i64)min(PB, n - j0) * ldab * ES), kQRsrcWord3);
 #pragma unroll
-            for (int s = 0; s < NS; ++s) o.a[s] = rp[(size_t)(wa * NS + s) * 64 + lane];
-        }
+        for (int s = 0; s < NS; ++s) o.a[s] = qbload<T>(rr, offA + s * 64 * ES);
 #pragma unroll
-        for (int m = 0; m < MT; ++m) {
-            const int t = w + m * NW;
+        for (int m = 0; m < MT; ++m)
 #pragma unroll
-            for (int s = 0; s < NS; ++s) {
-                const int i = j0 - top + 16 * t + jc;  // row of U
-                const int j = j0 + 4 * s + kq;         // column of U
-                const int d = kv + i - j;              // AB row; < kv since i < j0 <= j
-                const bool ok = t < KTU && i >= 0 && j < n && d >= 0;
-                const T v = ab[ok ? (i64)j * ldab + d : 0];
-                o.b[m][s] = ok ? v : T(0);
-            }
-        }
-        o.pn = 0;
+            for (int s = 0; s < NS; ++s) o.b[m][s] = qbload<T>(ru, offU[m][s]);
         const int frow = j0 - top - PB + fr;
-        const bool fok = fr < PB && frow >= 0 && c0 + fj < nrhs;
-        const T fv = B[(i64)min(c0 + fj, nrhs - 1) * ldb + max(frow, 0)];
-        o.fresh = fok ? fv : T(0);
+        o.fresh = qbload<T>(rB, (fr < PB && frow >= 0 && c0 + fj < nrhs) ? (fj * (int)ldb + frow) * ES : kQOob);
     };
 
-    // panel index q counts from the last panel (q = 0 is panel np-1); the
-    // same uniform load/store sequence as the forward sweep
+    // panel index q counts from the last panel (q = 0 is panel np-1)
     auto panel = [&](int q, const Ops &u, Ops &ld) {
         const int p = np - 1 - q;
         const int j0 = p * PB;
-        issue(max(p - 2, 0), ld);
         // X1 = U11^{-1} Y[j0:j0+PB]
-        v4 acc = {T(0), T(0), T(0), T(0)};
+        v4 x = {T(0), T(0), T(0), T(0)};
         if (w < NP) {
+            v4 x1 = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-            for (int s = 0; s < NS; ++s) acc = Mf<T>::mma(u.a[s], Yr[((j0 + 4 * s + kq) & msk) * SW + jc], acc);
+            for (int s = 0; s < NS; s += 2) {
+                x = Mf<T>::mma(u.a[s], Yr[((j0 + 4 * s + kq) & msk) * SW + jc], x);
+                x1 = Mf<T>::mma(u.a[s + 1], Yr[((j0 + 4 * s + 4 + kq) & msk) * SW + jc], x1);
+            }
+            x += x1;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) Xb[(16 * w + drow<T>(kq, v)) * SW + jc] = acc[v];
-        }
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const int r = 16 * w + drow<T>(kq, v);
-            const bool ok = w < NP && colok && j0 + r >= 0 && j0 + r < n;
-            qbstore<T>(rB, ok ? (jc * (int)ldb + j0 + r) * (int)sizeof(T) : kQOob, acc[v]);
+            for (int v = 0; v < 4; ++v) Xb[(16 * w + drow<T>(kq, v)) * SW + jc] = x[v];
         }
         lds_barrier();
+        issue(max(p - 2, 0), ld);
         // Y[j0-top : j0] -= U[j0-top : j0, j0 : j0+PB] X1
 #pragma unroll
         for (int m = 0; m < MT; ++m) {
             const int t = w + m * NW;
             if (t < KTU) {
-                v4 acc = {T(0), T(0), T(0), T(0)};
+                v4 z = {T(0), T(0), T(0), T(0)}, z1 = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-                for (int s = 0; s < NS; ++s) acc = Mf<T>::mma(u.b[m][s], Xb[(4 * s + kq) * SW + jc], acc);
+                for (int s = 0; s < NS; s += 2) {
+                    z = Mf<T>::mma(u.b[m][s], Xb[(4 * s + kq) * SW + jc], z);
+                    z1 = Mf<T>::mma(u.b[m][s + 1], Xb[(4 * s + 4 + kq) * SW + jc], z1);
+                }
+                z += z1;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int row = j0 - top + 16 * t + drow<T>(kq, v);
-                    if (row >= 0) Yr[(row & msk) * SW + jc] -= acc[v];
+                    if (row >= 0) Yr[(row & msk) * SW + jc] -= z[v];
                 }
             }
         }
         const int frow = j0 - top - PB + fr;
         if (fr < PB && frow >= 0) Yr[(frow & msk) * SW + fj] = u.fresh;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int r = 16 * w + drow<T>(kq, v);
+            const bool ok = w < NP && colok && j0 + r >= 0 && j0 + r < n;
+            qbstore<T>(rB, ok ? (jc * (int)ldb + j0 + r) * ES : kQOob, x[v]);
+        }
         lds_barrier();
     };
 
@@ -418,9 +440,8 @@ __global__ __launch_bounds__(64 * NW) void trs_bwd(T *B, i64 ldb, int nrhs, TrsL
     for (int e = tid; e < (top + PB) * 16; e += 64 * NW) {
         const int rr = e >> 4, j = e & 15;
         const int row = jl - top + rr;
-        T v = T(0);
-        if (row >= 0 && row < n && c0 + j < nrhs) v = B[(i64)(c0 + j) * ldb + row];
-        Yr[(row & msk) * SW + j] = v;
+        Yr[(row & msk) * SW + j] =
+            qbload<T>(rB, (row >= 0 && row < n && c0 + j < nrhs) ? (j * (int)ldb + row) * ES : kQOob);
     }
     lds_barrier();
     // panels before the first (at most two) touch only negative rows: their
@@ -445,13 +466,13 @@ size_t trs_blocked_bytes(i64 n, i64 kl, i64 ku) {
     constexpr int SW = TStride<T>::value;
     // 16 columns of B (leading dimension >= n) must fit one 31-bit buffer offset range
     if (n <= 0 || kl < 1 || 16 * n * (i64)sizeof(T) >= kQOob) return 0;
-    // update tiles per wave: at most 4 (FP32); FP64 registers hold 2 forward, 3 backward
-    const i64 mtf = sizeof(T) == 4 ? 4 : 2, mtb = sizeof(T) == 4 ? 4 : 3;
+    // update tiles per wave that the registers hold without spilling:
+    // FP32 3 forward, 4 backward; FP64 1 forward, 2 backward
+    const i64 mtf = sizeof(T) == 4 ? 3 : 1, mtb = sizeof(T) == 4 ? 4 : 2;
     if (kl > 16 * mtf * kTrsNW || kl + ku > 16 * mtb * kTrsNW) return 0;
     const Lt L = Lt::make(n, kl, ku);
-    if (L.RW > 64 * kTrsNW) return 0;  // one thread per window row carries the next permutation
     const int RWP = 16 * ((L.RW + 15) / 16);
-    const size_t fwd_lds = (size_t)(2 * RWP + kTrsPB) * SW * sizeof(T) + 2 * L.RW * sizeof(int);
+    const size_t fwd_lds = (size_t)(2 * RWP + kTrsPB) * SW * sizeof(T);
     int RB = 1;
     while (RB < 16 * L.KTU + 2 * kTrsPB) RB <<= 1;
     const size_t bwd_lds = (size_t)(RB + kTrsPB) * SW * sizeof(T);
@@ -494,17 +515,17 @@ int trs_blocked_solve(i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 ldab, co
     const int *perm = (const int *)((const char *)work + ((recs + 255) & ~(size_t)255));
     const dim3 grid((unsigned)((nrhs + 15) / 16));
     const int RWP = 16 * ((L.RW + 15) / 16);
-    const size_t fl = (size_t)(2 * RWP + kTrsPB) * SW * sizeof(T) + 2 * L.RW * sizeof(int);
+    const size_t fl = (size_t)(2 * RWP + kTrsPB) * SW * sizeof(T);
     const int mtf = (L.KT + kTrsNW - 1) / kTrsNW;
     void (*kf)(T *, i64, int, Lt, const int *, const T *) = nullptr;
     if constexpr (sizeof(T) == 4) {
+        if (mtf > 3) return BH_ERR_WORKSPACE;
         kf = mtf <= 1 ? trs_fwd<T, kTrsPB, kTrsNW, 1>
            : mtf == 2 ? trs_fwd<T, kTrsPB, kTrsNW, 2>
-           : mtf == 3 ? trs_fwd<T, kTrsPB, kTrsNW, 3>
-                      : trs_fwd<T, kTrsPB, kTrsNW, 4>;
+                      : trs_fwd<T, kTrsPB, kTrsNW, 3>;
     } else {
-        if (mtf > 2) return BH_ERR_WORKSPACE;
-        kf = mtf <= 1 ? trs_fwd<T, kTrsPB, kTrsNW, 1> : trs_fwd<T, kTrsPB, kTrsNW, 2>;
+        if (mtf > 1) return BH_ERR_WORKSPACE;
+        kf = trs_fwd<T, kTrsPB, kTrsNW, 1>;
     }
     if (fl > 64 * 1024) BH_HIP(hipFuncSetAttribute((const void *)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)fl));
     hipLaunchKernelGGL(kf, grid, dim3(64 * kTrsNW), fl, st, B, ldb, (int)nrhs, L, perm, rec);
@@ -520,10 +541,8 @@ int trs_blocked_solve(i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 ldab, co
            : mtb == 3 ? trs_bwd<T, kTrsPB, kTrsNW, 3>
                       : trs_bwd<T, kTrsPB, kTrsNW, 4>;
     } else {
-        if (mtb > 3) return BH_ERR_WORKSPACE;
-        kb = mtb <= 1 ? trs_bwd<T, kTrsPB, kTrsNW, 1>
-           : mtb == 2 ? trs_bwd<T, kTrsPB, kTrsNW, 2>
-                      : trs_bwd<T, kTrsPB, kTrsNW, 3>;
+        if (mtb > 2) return BH_ERR_WORKSPACE;
+        kb = mtb <= 1 ? trs_bwd<T, kTrsPB, kTrsNW, 1> : trs_bwd<T, kTrsPB, kTrsNW, 2>;
     }
     if (bl > 64 * 1024) BH_HIP(hipFuncSetAttribute((const void *)kb, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bl));
     hipLaunchKernelGGL(kb, grid, dim3(64 * kTrsNW), bl, st, B, ldb, (int)nrhs, L, ab, ldab, rec, RB);
@@ -555,8 +574,8 @@ static int gbtrs_blocked_impl(i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 
     if (n == 0 || nrhs == 0) return BH_OK;
     const size_t need = trs_blocked_bytes<T>(n, kl, ku);
     if (need == 0) {
-        set_error("gbtrs_blocked: bands (%lld,%lld) are outside the blocked solve's range (FP32: 1 <= kl <= 480, "
-                  "kl+ku <= 512; FP64: 1 <= kl <= 256, kl+ku <= 384); use bh_?gbtrs", (long long)kl, (long long)ku);
+        set_error("gbtrs_blocked: bands (%lld,%lld) are outside the blocked solve's range (FP32: 1 <= kl <= 384, "
+                  "kl+ku <= 512; FP64: 1 <= kl <= 128, kl+ku <= 256); use bh_?gbtrs", (long long)kl, (long long)ku);
         return BH_ERR_WORKSPACE;
     }
     if (work == nullptr || lwork < need) {

@@ -143,8 +143,8 @@ int bh_sgbtrs(char trans, bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const flo
  * panel's updates summed at once), not bit for bit; for many right-hand
  * sides with wide bands.  work: device workspace of
  * bh_gbtrs_blocked_workspace(prec 'd'/'s', ...) bytes (0: bands outside the
- * supported range -- FP32 1 <= kl <= 480, kl+ku <= 512; FP64 1 <= kl <= 256,
- * kl+ku <= 384 -- then BH_ERR_WORKSPACE).
+ * supported range -- FP32 1 <= kl <= 384, kl+ku <= 512; FP64 1 <= kl <= 128,
+ * kl+ku <= 256 -- then BH_ERR_WORKSPACE).
  * Same job as LAPACK.gbtrs! in ldiv!(::BandedLU, B) (src/banded/linalg.jl:24-30). */
 int bh_gbtrs_blocked_workspace(char prec, bh_int n, bh_int kl, bh_int ku, size_t *bytes);
 int bh_dgbtrs_blocked(bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const double *AB, bh_int ldab,
