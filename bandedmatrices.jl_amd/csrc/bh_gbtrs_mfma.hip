@@ -41,11 +41,17 @@ struct TStride {
     static constexpr int value = sizeof(T) == 4 ? 20 : 18;  // LDS row stride of [rows][16] tiles
 };
 
+// Per panel the pre-pass writes one record of A operands, each 16-row tile
+// lane-major (lane l's PB/4 K-chunk values contiguous, so a lane loads its
+// operands with 16-byte reads): L11^{-1} (PB/16 tiles), L21 (KT tiles),
+// U11^{-1} (PB/16 tiles), the U block above the panel (KTU tiles); and one
+// record of 16-bit window-row indices: the gather rows of the diagonal-block
+// product (4 x PB/4) and the C-operand rows of each L21 tile (KT x 16).
 template <typename T, int PB>
 struct TrsLayout {
     int n, kl, ku, kv, RW, KT, KTU, np;
-    // per-panel record offsets (elements of T) inside the pre-pass output
-    size_t linv_off, l21_off, uinv_off, per_panel, perm_ints;
+    size_t linv_off, l21_off, uinv_off, ublk_off, per_panel;  // elements of T
+    size_t cidx_off, idx_u16;                                 // 16-bit entries
     static TrsLayout make(i64 n, i64 kl, i64 ku) {
         TrsLayout L{};
         L.n = (int)n;
@@ -59,8 +65,10 @@ struct TrsLayout {
         L.linv_off = 0;
         L.l21_off = (size_t)PB * PB;
         L.uinv_off = L.l21_off + (size_t)L.KT * 16 * PB;
-        L.per_panel = L.uinv_off + (size_t)PB * PB;
-        L.perm_ints = (size_t)L.RW;
+        L.ublk_off = L.uinv_off + (size_t)PB * PB;
+        L.per_panel = L.ublk_off + (size_t)L.KTU * 16 * PB;
+        L.cidx_off = (size_t)PB;  // 4 x PB/4 gather rows first
+        L.idx_u16 = (L.cidx_off + (size_t)L.KT * 16 + 7) & ~(size_t)7;
         return L;
     }
 };
@@ -71,7 +79,8 @@ struct TrsLayout {
 // element (i, k)), so the solve kernels load them with one coalesced read each.
 template <typename T, int PB>
 __global__ __launch_bounds__(256) void trs_prep(const T *__restrict__ ab, i64 ldab, const i64 *__restrict__ ipiv,
-                                                TrsLayout<T, PB> L, int *__restrict__ permA, T *__restrict__ rec) {
+                                                TrsLayout<T, PB> L, unsigned short *__restrict__ idxA,
+                                                T *__restrict__ rec) {
     extern __shared__ __align__(16) unsigned char tp_raw[];
     constexpr int LS = PB + 1;
     const int RW = L.RW, n = L.n, kl = L.kl, kv = L.kv;
@@ -144,17 +153,30 @@ __global__ __launch_bounds__(256) void trs_prep(const T *__restrict__ ab, i64 ld
         }
     }
     __syncthreads();
+    constexpr int NS = PB / 4;
     T *out = rec + (size_t)p * L.per_panel;
+    // element e of a tile block: tile e / (64 NS), lane (e / NS) % 64, chunk e % NS
     for (int e = tid; e < PB * PB; e += 256) {
-        const int lane = e & 63, blk = e >> 6, s = blk % (PB / 4), ti = blk / (PB / 4);
+        const int s = e % NS, lane = (e / NS) & 63, ti = e / (64 * NS);
         out[L.linv_off + e] = Xi[(16 * ti + (lane & 15)) * LS + 4 * s + (lane >> 4)];
     }
     for (int e = tid; e < L.KT * 16 * PB; e += 256) {
-        const int lane = e & 63, blk = e >> 6, s = blk % (PB / 4), t = blk / (PB / 4);
+        const int s = e % NS, lane = (e / NS) & 63, t = e / (64 * NS);
         const int r = PB + 16 * t + (lane & 15);
         out[L.l21_off + e] = r < RW ? Lf[r * LS + 4 * s + (lane >> 4)] : T(0);
     }
-    for (int r = tid; r < RW; r += 256) permA[(size_t)p * L.perm_ints + r] = perm[r];
+    // U(i, j) above the panel: i = j0 - 16 KTU + 16t + lane%16, j = j0 + 4s + lane/16
+    for (int e = tid; e < L.KTU * 16 * PB; e += 256) {
+        const int s = e % NS, lane = (e / NS) & 63, t = e / (64 * NS);
+        const int i = j0 - 16 * L.KTU + 16 * t + (lane & 15), j = j0 + 4 * s + (lane >> 4), d = kv + i - j;
+        out[L.ublk_off + e] = (i >= 0 && j < n && d >= 0) ? ab[(i64)j * ldab + d] : T(0);
+    }
+    unsigned short *ix = idxA + (size_t)p * L.idx_u16;
+    for (int e = tid; e < PB; e += 256) ix[e] = (unsigned short)perm[4 * (e % NS) + e / NS];  // [kq][s]
+    for (int e = tid; e < L.KT * 16; e += 256) {
+        const int v = e & 3, kq = (e >> 2) & 3, t = e >> 4;
+        ix[L.cidx_off + e] = (unsigned short)perm[min(PB + 16 * t + drow<T>(kq, v), RW - 1)];
+    }
     __syncthreads();
     // U11^{-1}: column j by back substitution (thread j)
     if (tid < PB) {
@@ -171,7 +193,7 @@ __global__ __launch_bounds__(256) void trs_prep(const T *__restrict__ ab, i64 ld
     }
     __syncthreads();
     for (int e = tid; e < PB * PB; e += 256) {
-        const int lane = e & 63, blk = e >> 6, s = blk % (PB / 4), ti = blk / (PB / 4);
+        const int s = e % NS, lane = (e / NS) & 63, ti = e / (64 * NS);
         out[L.uinv_off + e] = Xi[(16 * ti + (lane & 15)) * LS + 4 * s + (lane >> 4)];
     }
 }
@@ -190,20 +212,42 @@ struct PanelOps {
     T fresh;       // one element of the rows entering the window
 };
 
-__device__ __forceinline__ int bload_i32(__amdgpu_buffer_rsrc_t r, int off) {
-    return (int)__builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0);
+typedef unsigned trs_v4u __attribute__((ext_vector_type(4)));
+typedef unsigned trs_v2u __attribute__((ext_vector_type(2)));
+
+// N consecutive elements at byte offset off (16-byte aligned) with 16-byte loads
+template <typename T, int N>
+__device__ __forceinline__ void bload_vec(__amdgpu_buffer_rsrc_t r, int off, T (&dst)[N]) {
+    static_assert((N * sizeof(T)) % 16 == 0, "whole 16-byte loads");
+#pragma unroll
+    for (int q = 0; q < (int)(N * sizeof(T) / 16); ++q) {
+        const trs_v4u x = __builtin_amdgcn_raw_buffer_load_b128(r, off + 16 * q, 0, 0);
+        if constexpr (sizeof(T) == 4) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                // through a scalar: __builtin_bit_cast of a vector element
+                // (x[i] directly) is lowered wrongly by this compiler
+                const unsigned e = x[i];
+                dst[4 * q + i] = __builtin_bit_cast(float, e);
+            }
+        } else {
+            dst[2 * q] = __builtin_bit_cast(double, (trs_v2u){x[0], x[1]});
+            dst[2 * q + 1] = __builtin_bit_cast(double, (trs_v2u){x[2], x[3]});
+        }
+    }
 }
 
 // ---------------------------------------------------------------- forward sweep
 // B (n x nrhs, ldb) <- L^{-1} P B for the 16 columns starting at 16*blockIdx.x.
 template <typename T, int PB, int NW, int MT>
-__global__ __launch_bounds__(64 * NW) void trs_fwd(T *B, i64 ldb, int nrhs, TrsLayout<T, PB> L, const int *permA,
-                                                   const T *rec) {
+__global__ __launch_bounds__(64 * NW) void trs_fwd(T *B, i64 ldb, int nrhs, TrsLayout<T, PB> L,
+                                                   const unsigned short *idxA, const T *rec) {
     using v4 = typename Mf<T>::v4;
     constexpr int SW = TStride<T>::value;
     constexpr int NS = PB / 4;   // K chunks of a panel product
     constexpr int NP = PB / 16;  // 16-row tiles of a panel (MT: L21 tiles per wave)
     constexpr int ES = (int)sizeof(T);
+    static_assert(NS == 8, "the gather rows of a lane are one 16-byte load");
     using Ops = PanelOps<T, NS, MT>;
     extern __shared__ __align__(16) unsigned char tf_raw[];
     const int RW = L.RW, KT = L.KT, n = L.n, np = L.np;
@@ -218,34 +262,38 @@ __global__ __launch_bounds__(64 * NW) void trs_fwd(T *B, i64 ldb, int nrhs, TrsL
     const bool colok = c0 + jc < nrhs;
     const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(B + (i64)c0 * ldb, 0, (int)(16 * ldb * ES),
                                                                         kQRsrcWord3);
-    // per-lane byte offsets inside a panel record and its permutation
-    const int offA = (int)((L.linv_off + (size_t)min(w, NP - 1) * NS * 64 + lane) * ES);
-    int offB[MT], rowC[MT][4];
+    // per-lane byte offsets inside a panel's records
+    const int offA = (int)((L.linv_off + ((size_t)min(w, NP - 1) * 64 + lane) * NS) * ES);
+    int offB[MT], offC[MT];
 #pragma unroll
     for (int m = 0; m < MT; ++m) {
         const int t = min(w + m * NW, KT - 1);
-        offB[m] = (int)((L.l21_off + (size_t)t * NS * 64 + lane) * ES);
-#pragma unroll
-        for (int v = 0; v < 4; ++v) rowC[m][v] = min(PB + 16 * t + drow<T>(kq, v), RW - 1);
+        offB[m] = (int)((L.l21_off + ((size_t)t * 64 + lane) * NS) * ES);
+        offC[m] = (int)((L.cidx_off + (size_t)(t * 4 + kq) * 4) * 2);
     }
 
     auto issue = [&](int p, Ops &o) {
         const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<T *>(rec) + (size_t)p * L.per_panel, 0, (int)(L.per_panel * ES), kQRsrcWord3);
-        const __amdgpu_buffer_rsrc_t rp = __builtin_amdgcn_make_buffer_rsrc(
-            const_cast<int *>(permA) + (size_t)p * L.perm_ints, 0, RW * 4, kQRsrcWord3);
+        const __amdgpu_buffer_rsrc_t ri = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<unsigned short *>(idxA) + (size_t)p * L.idx_u16, 0, (int)(L.idx_u16 * 2), kQRsrcWord3);
+        bload_vec<T, NS>(rr, offA, o.a);
 #pragma unroll
-        for (int s = 0; s < NS; ++s) o.a[s] = qbload<T>(rr, offA + s * 64 * ES);
+        for (int m = 0; m < MT; ++m) bload_vec<T, NS>(rr, offB[m], o.b[m]);
+        const trs_v4u gi = __builtin_amdgcn_raw_buffer_load_b128(ri, kq * NS * 2, 0, 0);
 #pragma unroll
-        for (int m = 0; m < MT; ++m)
+        for (int q = 0; q < 4; ++q) {
+            o.g[2 * q] = (int)(gi[q] & 0xffffu);
+            o.g[2 * q + 1] = (int)(gi[q] >> 16);
+        }
 #pragma unroll
-            for (int s = 0; s < NS; ++s) o.b[m][s] = qbload<T>(rr, offB[m] + s * 64 * ES);
-#pragma unroll
-        for (int s = 0; s < NS; ++s) o.g[s] = bload_i32(rp, (4 * s + kq) * 4);
-#pragma unroll
-        for (int m = 0; m < MT; ++m)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) o.c[m][v] = bload_i32(rp, rowC[m][v] * 4);
+        for (int m = 0; m < MT; ++m) {
+            const trs_v2u ci = __builtin_amdgcn_raw_buffer_load_b64(ri, offC[m], 0, 0);
+            o.c[m][0] = (int)(ci[0] & 0xffffu);
+            o.c[m][1] = (int)(ci[0] >> 16);
+            o.c[m][2] = (int)(ci[1] & 0xffffu);
+            o.c[m][3] = (int)(ci[1] >> 16);
+        }
         const int row = p * PB + RW + fr;
         o.fresh = qbload<T>(rB, (fr < PB && row < n && c0 + fj < nrhs) ? (fj * (int)ldb + row) * ES : kQOob);
     };
@@ -326,10 +374,10 @@ __global__ __launch_bounds__(64 * NW) void trs_fwd(T *B, i64 ldb, int nrhs, TrsL
 }
 
 // ---------------------------------------------------------------- backward sweep
-// B <- U^{-1} B (U: upper band of width kv = kl+ku in AB) for 16 columns.
+// B <- U^{-1} B (U: upper band of width kv = kl+ku) for 16 columns; the U
+// operands come from the pre-pass records.
 template <typename T, int PB, int NW, int MT>
-__global__ __launch_bounds__(64 * NW) void trs_bwd(T *B, i64 ldb, int nrhs, TrsLayout<T, PB> L, const T *ab, i64 ldab,
-                                                   const T *rec, int RB) {
+__global__ __launch_bounds__(64 * NW) void trs_bwd(T *B, i64 ldb, int nrhs, TrsLayout<T, PB> L, const T *rec, int RB) {
     using v4 = typename Mf<T>::v4;
     constexpr int SW = TStride<T>::value;
     constexpr int NS = PB / 4;
@@ -337,7 +385,7 @@ __global__ __launch_bounds__(64 * NW) void trs_bwd(T *B, i64 ldb, int nrhs, TrsL
     constexpr int ES = (int)sizeof(T);
     using Ops = PanelOps<T, NS, MT>;
     extern __shared__ __align__(16) unsigned char tb_raw[];
-    const int n = L.n, kv = L.kv, KTU = L.KTU, np = L.np;
+    const int n = L.n, KTU = L.KTU, np = L.np;
     const int top = 16 * KTU;  // rows above a panel that it updates (>= kv)
     T *Yr = reinterpret_cast<T *>(tb_raw);  // ring [RB][SW], row i in slot i & (RB-1)
     T *Xb = Yr + RB * SW;                   // [PB][SW]
@@ -349,36 +397,19 @@ __global__ __launch_bounds__(64 * NW) void trs_bwd(T *B, i64 ldb, int nrhs, TrsL
     const bool colok = c0 + jc < nrhs;
     const __amdgpu_buffer_rsrc_t rB = __builtin_amdgcn_make_buffer_rsrc(B + (i64)c0 * ldb, 0, (int)(16 * ldb * ES),
                                                                         kQRsrcWord3);
-    const int offA = (int)((L.uinv_off + (size_t)min(w, NP - 1) * NS * 64 + lane) * ES);
-    // U(i, j) for i = j0 - top + 16t + jc, j = j0 + 4s + kq sits at element
-    // (4s+kq) ldab + d of the panel's columns, d = kv + i - j the AB row;
-    // d < 0 is outside the band (offset sent out of range: reads 0).  Rows
-    // i < 0 read corner slots whose products land in rows that are skipped.
-    const int li = (int)ldab;
-    int offU[MT][NS];
+    const int offA = (int)((L.uinv_off + ((size_t)min(w, NP - 1) * 64 + lane) * NS) * ES);
+    int offB[MT];
 #pragma unroll
-    for (int m = 0; m < MT; ++m) {
-        const int t = min(w + m * NW, KTU - 1);
-#pragma unroll
-        for (int s = 0; s < NS; ++s) {
-            const int d = kv - top + 16 * t + jc - 4 * s - kq;
-            offU[m][s] = d >= 0 ? ((4 * s + kq) * li + d) * ES : kQOob;
-        }
-    }
+    for (int m = 0; m < MT; ++m)
+        offB[m] = (int)((L.ublk_off + ((size_t)min(w + m * NW, KTU - 1) * 64 + lane) * NS) * ES);
 
     auto issue = [&](int p, Ops &o) {
         const int j0 = p * PB;
         const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
             const_cast<T *>(rec) + (size_t)p * L.per_panel, 0, (int)(L.per_panel * ES), kQRsrcWord3);
-        // the panel's columns, up to column n (columns past it read 0)
-        const __amdgpu_buffer_rsrc_t ru = __builtin_amdgcn_make_buffer_rsrc(
-            const_cast<T *>(ab) + (i64)j0 * ldab, 0, (int)((i64)min(PB, n - j0) * ldab * ES), kQRsrcWord3);
+        bload_vec<T, NS>(rr, offA, o.a);
 #pragma unroll
-        for (int s = 0; s < NS; ++s) o.a[s] = qbload<T>(rr, offA + s * 64 * ES);
-#pragma unroll
-        for (int m = 0; m < MT; ++m)
-#pragma unroll
-            for (int s = 0; s < NS; ++s) o.b[m][s] = qbload<T>(ru, offU[m][s]);
+        for (int m = 0; m < MT; ++m) bload_vec<T, NS>(rr, offB[m], o.b[m]);
         const int frow = j0 - top - PB + fr;
         o.fresh = qbload<T>(rB, (fr < PB && frow >= 0 && c0 + fj < nrhs) ? (fj * (int)ldb + frow) * ES : kQOob);
     };
@@ -479,8 +510,9 @@ size_t trs_blocked_bytes(i64 n, i64 kl, i64 ku) {
     const size_t prep_lds = ((size_t)L.RW * (kTrsPB + 1) + 2 * kTrsPB * (kTrsPB + 1)) * sizeof(T) + 2 * L.RW * sizeof(int);
     if (fwd_lds > 160 * 1024 || bwd_lds > 160 * 1024 || prep_lds > 160 * 1024) return 0;
     const size_t recs = (size_t)L.np * L.per_panel * sizeof(T);
-    const size_t perms = (size_t)L.np * L.perm_ints * sizeof(int);
-    return ((recs + 255) & ~(size_t)255) + ((perms + 255) & ~(size_t)255);
+    const size_t idxs = (size_t)L.np * L.idx_u16 * 2;
+    if (L.per_panel * sizeof(T) >= (size_t)kQOob) return 0;
+    return ((recs + 255) & ~(size_t)255) + ((idxs + 255) & ~(size_t)255);
 }
 
 template <typename T>
@@ -489,12 +521,12 @@ int trs_blocked_prepare(i64 n, i64 kl, i64 ku, const T *ab, i64 ldab, const i64 
     const Lt L = Lt::make(n, kl, ku);
     const size_t recs = (size_t)L.np * L.per_panel * sizeof(T);
     T *rec = (T *)work;
-    int *perm = (int *)((char *)work + ((recs + 255) & ~(size_t)255));
+    unsigned short *idx = (unsigned short *)((char *)work + ((recs + 255) & ~(size_t)255));
     const size_t lds = ((size_t)L.RW * (kTrsPB + 1) + 2 * kTrsPB * (kTrsPB + 1)) * sizeof(T) + 2 * L.RW * sizeof(int);
     auto kern = trs_prep<T, kTrsPB>;
     if (lds > 64 * 1024)
         BH_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    hipLaunchKernelGGL(kern, dim3((unsigned)L.np), dim3(256), lds, st, ab, ldab, ipiv, L, perm, rec);
+    hipLaunchKernelGGL(kern, dim3((unsigned)L.np), dim3(256), lds, st, ab, ldab, ipiv, L, idx, rec);
     BH_LAUNCH_CHECK();
     return BH_OK;
 }
@@ -512,12 +544,12 @@ int trs_blocked_solve(i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 ldab, co
     const Lt L = Lt::make(n, kl, ku);
     const size_t recs = (size_t)L.np * L.per_panel * sizeof(T);
     const T *rec = (const T *)work;
-    const int *perm = (const int *)((const char *)work + ((recs + 255) & ~(size_t)255));
+    const unsigned short *idx = (const unsigned short *)((const char *)work + ((recs + 255) & ~(size_t)255));
     const dim3 grid((unsigned)((nrhs + 15) / 16));
     const int RWP = 16 * ((L.RW + 15) / 16);
     const size_t fl = (size_t)(2 * RWP + kTrsPB) * SW * sizeof(T);
     const int mtf = (L.KT + kTrsNW - 1) / kTrsNW;
-    void (*kf)(T *, i64, int, Lt, const int *, const T *) = nullptr;
+    void (*kf)(T *, i64, int, Lt, const unsigned short *, const T *) = nullptr;
     if constexpr (sizeof(T) == 4) {
         if (mtf > 3) return BH_ERR_WORKSPACE;
         kf = mtf <= 1 ? trs_fwd<T, kTrsPB, kTrsNW, 1>
@@ -528,13 +560,13 @@ int trs_blocked_solve(i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 ldab, co
         kf = trs_fwd<T, kTrsPB, kTrsNW, 1>;
     }
     if (fl > 64 * 1024) BH_HIP(hipFuncSetAttribute((const void *)kf, hipFuncAttributeMaxDynamicSharedMemorySize, (int)fl));
-    hipLaunchKernelGGL(kf, grid, dim3(64 * kTrsNW), fl, st, B, ldb, (int)nrhs, L, perm, rec);
+    hipLaunchKernelGGL(kf, grid, dim3(64 * kTrsNW), fl, st, B, ldb, (int)nrhs, L, idx, rec);
     BH_LAUNCH_CHECK();
     int RB = 1;
     while (RB < 16 * L.KTU + 2 * kTrsPB) RB <<= 1;
     const size_t bl = (size_t)(RB + kTrsPB) * SW * sizeof(T);
     const int mtb = (L.KTU + kTrsNW - 1) / kTrsNW;
-    void (*kb)(T *, i64, int, Lt, const T *, i64, const T *, int) = nullptr;
+    void (*kb)(T *, i64, int, Lt, const T *, int) = nullptr;
     if constexpr (sizeof(T) == 4) {
         kb = mtb <= 1 ? trs_bwd<T, kTrsPB, kTrsNW, 1>
            : mtb == 2 ? trs_bwd<T, kTrsPB, kTrsNW, 2>
@@ -545,7 +577,7 @@ int trs_blocked_solve(i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 ldab, co
         kb = mtb <= 1 ? trs_bwd<T, kTrsPB, kTrsNW, 1> : trs_bwd<T, kTrsPB, kTrsNW, 2>;
     }
     if (bl > 64 * 1024) BH_HIP(hipFuncSetAttribute((const void *)kb, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bl));
-    hipLaunchKernelGGL(kb, grid, dim3(64 * kTrsNW), bl, st, B, ldb, (int)nrhs, L, ab, ldab, rec, RB);
+    hipLaunchKernelGGL(kb, grid, dim3(64 * kTrsNW), bl, st, B, ldb, (int)nrhs, L, rec, RB);
     BH_LAUNCH_CHECK();
     return BH_OK;
 }
