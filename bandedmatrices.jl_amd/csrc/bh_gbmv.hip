@@ -145,6 +145,64 @@ gbmv_n_direct(i64 m, i64 n, i64 kl, i64 ku, i64 nrhs, T alpha, const T *__restri
     }
 }
 
+// -------------------------------------------------------------- 'N' wide
+// Bands too wide for the slab, several right-hand sides: a workgroup owns 64
+// rows x 16 right-hand sides and walks the band's columns in chunks of 16,
+// staging the 64 x 16 block of A (each column's 64 rows are contiguous in
+// band storage) and alpha*x for the chunk in LDS; a thread keeps 4
+// right-hand sides of one row.  Every y_i still accumulates fma(alpha x_j,
+// a_ij, acc) over ascending j, skipping entries outside the band, exactly
+// as the direct kernel (and OpenBLAS's column sweep) does.
+template <typename T>
+__global__ void __launch_bounds__(256)
+gbmv_n_wide(i64 m, i64 n, i64 kl, i64 ku, i64 nrhs, T alpha, const T *__restrict__ A, i64 lda,
+            const T *__restrict__ X, i64 incx, i64 ldx, T beta, T *__restrict__ Y, i64 incy, i64 ldy) {
+    __shared__ T As[16][65];  // [column in chunk][row]
+    __shared__ T Xs[16][17];  // [column in chunk][right-hand side]: alpha * x
+    const int tid = threadIdx.x, rr = tid & 63, rg = tid >> 6;  // row, group of 4 right-hand sides
+    const i64 r0 = (i64)blockIdx.x * 64, c0 = (i64)blockIdx.y * 16;
+    const i64 i = r0 + rr;
+    const i64 h = kl + ku;
+    T acc[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const i64 r = c0 + 4 * rg + q;
+        acc[q] = (i < m && r < nrhs && beta != T(0)) ? beta * Y[r * ldy + i * incy] : T(0);
+    }
+    const i64 jlo = imax(0, r0 - kl), jhi = imin(n - 1, r0 + 63 + ku);
+    for (i64 jb = jlo; jb <= jhi; jb += 16) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {  // 64 x 16 block of A, one column per 64 threads
+            const int jj = rg + 4 * q;
+            const i64 j = jb + jj, t = ku + i - j;
+            As[jj][rr] = (j <= jhi && t >= 0 && t <= h && i < m) ? A[j * lda + t] : T(0);
+        }
+        {
+            const int jj = tid >> 4, r = tid & 15;
+            const i64 j = jb + jj;
+            Xs[jj][r] = (j <= jhi && c0 + r < nrhs) ? alpha * X[(c0 + r) * ldx + j * incx] : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int jj = 0; jj < 16; ++jj) {
+            const i64 j = jb + jj, t = ku + i - j;
+            if (j <= jhi && t >= 0 && t <= h) {
+                const T a = As[jj][rr];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) acc[q] = dfma(Xs[jj][4 * rg + q], a, acc[q]);
+            }
+        }
+        __syncthreads();
+    }
+    if (i < m) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const i64 r = c0 + 4 * rg + q;
+            if (r < nrhs) Y[r * ldy + i * incy] = acc[q];
+        }
+    }
+}
+
 // ----------------------------------------------------------------- 'T' tile
 // y_j = beta*y_j + alpha * sum_i a_ij x_i: one thread per column j, the dot
 // accumulated in ascending i from the LDS slab (columns are contiguous).
@@ -274,6 +332,10 @@ static int gbmv_impl(char trans, i64 m, i64 n, i64 kl, i64 ku, i64 nrhs, T alpha
         else
             hipLaunchKernelGGL(gbmv_t_tile<T>, grid, dim3(kTileThreads), lds, s, m, n, kl, ku, nrhs, alpha, A, lda,
                                X, incx, ldx, beta, Y, incy, ldy);
+    } else if (nt && nrhs > 1) {
+        dim3 grid((unsigned)cdiv(m, 64), (unsigned)cdiv(nrhs, 16));
+        hipLaunchKernelGGL(gbmv_n_wide<T>, grid, dim3(256), 0, s, m, n, kl, ku, nrhs, alpha, A, lda, X, incx, ldx,
+                           beta, Y, incy, ldy);
     } else if (nt) {
         dim3 grid((unsigned)cdiv(cdiv(m, 64), 4));
         hipLaunchKernelGGL(gbmv_n_direct<T>, grid, dim3(256), 0, s, m, n, kl, ku, nrhs, alpha, A, lda, X, incx, ldx,

@@ -14,21 +14,26 @@
 // (kl x PB))
 //     Y1 = L11^{-1} (perm W)[0:PB]            the panel's finished rows
 //     W' = (perm W)[PB:RW] - L21 Y1           the rows the next panel sees
-// perm, L11^{-1} and L21 depend on the factors only; a parallel pre-pass
+//        = (perm W)[PB:RW] - M21 (perm W)[0:PB],   M21 = L21 L11^{-1}
+// perm, L11^{-1} and M21 depend on the factors only; a parallel pre-pass
 // (one workgroup per panel) builds them once per factorisation.  L21 in
 // final order is xGBTRF's form of the multipliers before it "partially
 // undoes" the interchanges: each stored column c is moved to the positions
 // its rows reach after the panel's later interchanges.
-// Backward.  Panels from the last: X1 = U11^{-1} Y[j0:j0+PB]; then the kv =
-// kl+ku rows above the panel lose U[j0-kv:j0, j0:j0+PB] X1.
+// Backward.  Panels from the last: X1 = U11^{-1} Y[j0:j0+PB], and the kv =
+// kl+ku rows above the panel lose U[j0-kv:j0, j0:j0+PB] X1 =
+// V Y[j0:j0+PB], V = U[j0-kv:j0, j0:j0+PB] U11^{-1}.
 //
-// One workgroup solves 16 right-hand sides; each link is two MFMA products
-// and two LDS-only barriers; the next panel's operands are prefetched into
-// registers during the current one.  Results equal xGBTRS's to rounding
-// (the panel's updates reach a row as one sum, the diagonal blocks through
-// their inverses), not bit for bit: this solve serves the mixed-precision
-// refinement, whose corrections need an accurate solve, not LAPACK's
-// operation order; bh_?gbtrs stays the bit-exact path.
+// Both products of a link read only the window as it stands, so they run
+// side by side: one workgroup solves 16 right-hand sides with one MFMA
+// product per 16-row tile and one LDS-only barrier per link; the operands
+// are loaded two links ahead.  Results equal xGBTRS's to rounding (the
+// diagonal blocks through their inverses, a panel's updates in one sum),
+// not bit for bit: this solve serves the mixed-precision refinement, whose
+// corrections need an accurate solve, not LAPACK's operation order;
+// bh_?gbtrs stays the bit-exact path.
+#include <type_traits>
+
 #include "bh_common.hpp"
 #include "bh_mfma.hpp"
 
@@ -43,10 +48,12 @@ struct TStride {
 
 // Per panel the pre-pass writes one record of A operands, each 16-row tile
 // lane-major (lane l's PB/4 K-chunk values contiguous, so a lane loads its
-// operands with 16-byte reads): L11^{-1} (PB/16 tiles), L21 (KT tiles),
-// U11^{-1} (PB/16 tiles), the U block above the panel (KTU tiles); and one
-// record of 16-bit window-row indices: the gather rows of the diagonal-block
-// product (4 x PB/4) and the C-operand rows of each L21 tile (KT x 16).
+// operands with 16-byte reads): L11^{-1} (PB/16 tiles), M21 (KT tiles),
+// U11^{-1} (PB/16 tiles), -V (KTU tiles) -- the update blocks negated, so
+// the sweeps accumulate onto the rows' current values; and one record of
+// 16-bit window rows, as byte offsets into the forward sweep's LDS window:
+// the gather rows of the diagonal-block product (4 x PB/4) and the C-operand
+// rows of each M21 tile (KT x 16).
 template <typename T, int PB>
 struct TrsLayout {
     int n, kl, ku, kv, RW, KT, KTU, np;
@@ -84,10 +91,12 @@ __global__ __launch_bounds__(256) void trs_prep(const T *__restrict__ ab, i64 ld
     extern __shared__ __align__(16) unsigned char tp_raw[];
     constexpr int LS = PB + 1;
     const int RW = L.RW, n = L.n, kl = L.kl, kv = L.kv;
+    const int top = 16 * L.KTU;
     T *Lf = reinterpret_cast<T *>(tp_raw);  // RW x LS
     T *Ut = Lf + (size_t)RW * LS;           // PB x LS
     T *Xi = Ut + PB * LS;                   // PB x LS
-    int *perm = reinterpret_cast<int *>(Xi + PB * LS);
+    T *Ub = Xi + PB * LS;                   // top x LS: U block above the panel
+    int *perm = reinterpret_cast<int *>(Ub + (size_t)top * LS);
     int *F = perm + RW;
     __shared__ int q[PB];
     const int p = blockIdx.x, tid = threadIdx.x;
@@ -160,22 +169,28 @@ __global__ __launch_bounds__(256) void trs_prep(const T *__restrict__ ab, i64 ld
         const int s = e % NS, lane = (e / NS) & 63, ti = e / (64 * NS);
         out[L.linv_off + e] = Xi[(16 * ti + (lane & 15)) * LS + 4 * s + (lane >> 4)];
     }
+    // -M21 = -L21 L11^{-1} (L11^{-1} lower triangular)
     for (int e = tid; e < L.KT * 16 * PB; e += 256) {
         const int s = e % NS, lane = (e / NS) & 63, t = e / (64 * NS);
-        const int r = PB + 16 * t + (lane & 15);
-        out[L.l21_off + e] = r < RW ? Lf[r * LS + 4 * s + (lane >> 4)] : T(0);
+        const int r = PB + 16 * t + (lane & 15), c = 4 * s + (lane >> 4);
+        T acc = T(0);
+        if (r < RW)
+            for (int k = c; k < PB; ++k) acc = dfma(Lf[r * LS + k], Xi[k * LS + c], acc);
+        out[L.l21_off + e] = -acc;  // stored negated: the sweep accumulates onto the C operand
     }
-    // U(i, j) above the panel: i = j0 - 16 KTU + 16t + lane%16, j = j0 + 4s + lane/16
-    for (int e = tid; e < L.KTU * 16 * PB; e += 256) {
-        const int s = e % NS, lane = (e / NS) & 63, t = e / (64 * NS);
-        const int i = j0 - 16 * L.KTU + 16 * t + (lane & 15), j = j0 + 4 * s + (lane >> 4), d = kv + i - j;
-        out[L.ublk_off + e] = (i >= 0 && j < n && d >= 0) ? ab[(i64)j * ldab + d] : T(0);
+    // U(i, j) above the panel: i = j0 - top + r, j = j0 + c
+    for (int e = tid; e < top * PB; e += 256) {
+        const int r = e / PB, c = e - r * PB;
+        const int i = j0 - top + r, j = j0 + c, d = kv + i - j;
+        Ub[r * LS + c] = (i >= 0 && j < n && d >= 0) ? ab[(i64)j * ldab + d] : T(0);
     }
     unsigned short *ix = idxA + (size_t)p * L.idx_u16;
-    for (int e = tid; e < PB; e += 256) ix[e] = (unsigned short)perm[4 * (e % NS) + e / NS];  // [kq][s]
+    // window rows as LDS byte offsets of the sweep's [rows][SW] buffer (< 2^16)
+    constexpr int RB_ = TStride<T>::value * (int)sizeof(T);
+    for (int e = tid; e < PB; e += 256) ix[e] = (unsigned short)(perm[4 * (e % NS) + e / NS] * RB_);  // [kq][s]
     for (int e = tid; e < L.KT * 16; e += 256) {
         const int v = e & 3, kq = (e >> 2) & 3, t = e >> 4;
-        ix[L.cidx_off + e] = (unsigned short)perm[min(PB + 16 * t + drow<T>(kq, v), RW - 1)];
+        ix[L.cidx_off + e] = (unsigned short)(perm[min(PB + 16 * t + drow<T>(kq, v), RW - 1)] * RB_);
     }
     __syncthreads();
     // U11^{-1}: column j by back substitution (thread j)
@@ -195,6 +210,14 @@ __global__ __launch_bounds__(256) void trs_prep(const T *__restrict__ ab, i64 ld
     for (int e = tid; e < PB * PB; e += 256) {
         const int s = e % NS, lane = (e / NS) & 63, ti = e / (64 * NS);
         out[L.uinv_off + e] = Xi[(16 * ti + (lane & 15)) * LS + 4 * s + (lane >> 4)];
+    }
+    // V = U_block U11^{-1} (U11^{-1} upper triangular), stored negated
+    for (int e = tid; e < L.KTU * 16 * PB; e += 256) {
+        const int s = e % NS, lane = (e / NS) & 63, t = e / (64 * NS);
+        const int r = 16 * t + (lane & 15), c = 4 * s + (lane >> 4);
+        T acc = T(0);
+        for (int k = 0; k <= c; ++k) acc = dfma(Ub[r * LS + k], Xi[k * LS + c], acc);
+        out[L.ublk_off + e] = -acc;
     }
 }
 
@@ -239,21 +262,23 @@ __device__ __forceinline__ void bload_vec(__amdgpu_buffer_rsrc_t r, int off, T (
 
 // ---------------------------------------------------------------- forward sweep
 // B (n x nrhs, ldb) <- L^{-1} P B for the 16 columns starting at 16*blockIdx.x.
+// LDS addresses are precomputed per lane; the window buffer a panel reads
+// (cur) is a compile-time constant of each of the six unrolled panels.
 template <typename T, int PB, int NW, int MT>
 __global__ __launch_bounds__(64 * NW) void trs_fwd(T *B, i64 ldb, int nrhs, TrsLayout<T, PB> L,
                                                    const unsigned short *idxA, const T *rec) {
     using v4 = typename Mf<T>::v4;
     constexpr int SW = TStride<T>::value;
     constexpr int NS = PB / 4;   // K chunks of a panel product
-    constexpr int NP = PB / 16;  // 16-row tiles of a panel (MT: L21 tiles per wave)
+    constexpr int NP = PB / 16;  // 16-row tiles of a panel (MT: M21 tiles per wave)
     constexpr int ES = (int)sizeof(T);
+    constexpr int RS = SW * ES;  // LDS bytes per window row
     static_assert(NS == 8, "the gather rows of a lane are one 16-byte load");
     using Ops = PanelOps<T, NS, MT>;
     extern __shared__ __align__(16) unsigned char tf_raw[];
     const int RW = L.RW, KT = L.KT, n = L.n, np = L.np;
-    const int RWP = 16 * ((RW + 15) / 16);
-    T *W = reinterpret_cast<T *>(tf_raw);  // [2][RWP][SW]
-    T *Yb = W + 2 * RWP * SW;              // [PB][SW]
+    const int RWP = 16 * ((RW + 15) / 16) + 1;  // + one dump row for results past RW
+    const int WB = RWP * RS;                    // bytes of one window buffer
     // w is wave-uniform (branches on it stay scalar)
     const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int jc = lane & 15, kq = lane >> 4;
@@ -264,13 +289,25 @@ __global__ __launch_bounds__(64 * NW) void trs_fwd(T *B, i64 ldb, int nrhs, TrsL
                                                                         kQRsrcWord3);
     // per-lane byte offsets inside a panel's records
     const int offA = (int)((L.linv_off + ((size_t)min(w, NP - 1) * 64 + lane) * NS) * ES);
-    int offB[MT], offC[MT];
+    int offB[MT], offC[MT], wdst[MT][4];
 #pragma unroll
     for (int m = 0; m < MT; ++m) {
         const int t = min(w + m * NW, KT - 1);
         offB[m] = (int)((L.l21_off + ((size_t)t * 64 + lane) * NS) * ES);
         offC[m] = (int)((L.cidx_off + (size_t)(t * 4 + kq) * 4) * 2);
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {  // where row r = PB + 16t + drow lands in the next window
+            const int r = PB + 16 * (w + m * NW) + drow<T>(kq, v);
+            wdst[m][v] = (r < RW ? r - PB : RWP - 1) * RS + jc * ES;
+        }
     }
+    const int fdst = fr < PB ? (RW - PB + fr) * RS + fj * ES : (RWP - 1) * RS + fj * ES;
+    // Y1 rows of this lane relative to the panel (stores of waves past NP
+    // and of columns past nrhs go out of range)
+    int ysto[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+        ysto[v] = (w < NP && colok) ? (jc * (int)ldb + 16 * w + drow<T>(kq, v)) * ES : kQOob;
 
     auto issue = [&](int p, Ops &o) {
         const __amdgpu_buffer_rsrc_t rr = __builtin_amdgcn_make_buffer_rsrc(
@@ -298,59 +335,46 @@ __global__ __launch_bounds__(64 * NW) void trs_fwd(T *B, i64 ldb, int nrhs, TrsL
         o.fresh = qbload<T>(rB, (fr < PB && row < n && c0 + fj < nrhs) ? (fj * (int)ldb + row) * ES : kQOob);
     };
 
+    unsigned char *lds = tf_raw;
     // one panel: uses `u`, starts the loads of panel p+2 into `ld`
-    auto panel = [&](int p, const Ops &u, Ops &ld) {
-        const int cur = p & 1;
+    auto panel = [&](int p, auto curc, const Ops &u, Ops &ld) {
+        constexpr int cur = decltype(curc)::value;
         const int j0 = p * PB;
-        const T *Wc = W + cur * RWP * SW;
-        T *Wn = W + (cur ^ 1) * RWP * SW;
-        // Y1 = L11^{-1} (perm W)[0:PB], two interleaved accumulations
-        v4 y = {T(0), T(0), T(0), T(0)};
-        if (w < NP) {
-            v4 y1 = {T(0), T(0), T(0), T(0)};
-#pragma unroll
-            for (int s = 0; s < NS; s += 2) {
-                y = Mf<T>::mma(u.a[s], Wc[u.g[s] * SW + jc], y);
-                y1 = Mf<T>::mma(u.a[s + 1], Wc[u.g[s + 1] * SW + jc], y1);
-            }
-            y += y1;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) Yb[(16 * w + drow<T>(kq, v)) * SW + jc] = y[v];
-        }
-        // C operands of the update tiles (independent of Y1)
-        T cin[MT][4];
-#pragma unroll
-        for (int m = 0; m < MT; ++m)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) cin[m][v] = Wc[u.c[m][v] * SW + jc];
-        lds_barrier();
+        const unsigned char *Wc = lds + cur * WB + jc * ES;
+        unsigned char *Wn = lds + (cur ^ 1) * WB;
         issue(min(p + 2, np - 1), ld);
-        // W' = (perm W)[PB:RW] - L21 Y1
+        // the gathered rows (perm W)[0:PB] as B operands
+        T g0[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) g0[s] = *reinterpret_cast<const T *>(Wc + u.g[s]);
+        // W' = (perm W)[PB:RW] + (-M21) (perm W)[0:PB]
 #pragma unroll
         for (int m = 0; m < MT; ++m) {
             const int t = w + m * NW;
             if (t < KT) {
-                v4 z = {T(0), T(0), T(0), T(0)}, z1 = {T(0), T(0), T(0), T(0)};
+                v4 z;
 #pragma unroll
-                for (int s = 0; s < NS; s += 2) {
-                    z = Mf<T>::mma(u.b[m][s], Yb[(4 * s + kq) * SW + jc], z);
-                    z1 = Mf<T>::mma(u.b[m][s + 1], Yb[(4 * s + 4 + kq) * SW + jc], z1);
-                }
-                z += z1;
+                for (int v = 0; v < 4; ++v) z[v] = *reinterpret_cast<const T *>(Wc + u.c[m][v]);
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int r = PB + 16 * t + drow<T>(kq, v);
-                    if (r < RW) Wn[(r - PB) * SW + jc] = cin[m][v] - z[v];
-                }
+                for (int s = 0; s < NS; ++s) z = Mf<T>::mma(u.b[m][s], g0[s], z);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) *reinterpret_cast<T *>(Wn + wdst[m][v]) = z[v];
             }
         }
-        if (fr < PB) Wn[(RW - PB + fr) * SW + fj] = u.fresh;
-        // Y1 to B (rows past n, columns past nrhs and waves past NP: dropped)
+        *reinterpret_cast<T *>(Wn + fdst) = u.fresh;
+        // Y1 = L11^{-1} (perm W)[0:PB], to B (rows past n, columns past
+        // nrhs and waves past NP: dropped)
+        v4 y = {T(0), T(0), T(0), T(0)};
+        if (w < NP) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) y = Mf<T>::mma(u.a[s], g0[s], y);
+        }
+        const bool full = j0 + PB <= n;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int r = 16 * w + drow<T>(kq, v);
-            const bool ok = w < NP && colok && j0 + r < n;
-            qbstore<T>(rB, ok ? (jc * (int)ldb + j0 + r) * ES : kQOob, y[v]);
+            const int off = (full || j0 + r < n) && ysto[v] != kQOob ? ysto[v] + j0 * ES : kQOob;
+            qbstore<T>(rB, off, y[v]);
         }
         lds_barrier();
     };
@@ -358,18 +382,24 @@ __global__ __launch_bounds__(64 * NW) void trs_fwd(T *B, i64 ldb, int nrhs, TrsL
     Ops S0, S1, S2;
     issue(0, S0);
     issue(min(1, np - 1), S1);
-    // window of panel 0
-    for (int e = tid; e < RWP * 16; e += 64 * NW) {
+    // window of panel 0 (buffer 0)
+    for (int e = tid; e < (RWP - 1) * 16; e += 64 * NW) {
         const int r = e >> 4, j = e & 15;
-        W[r * SW + j] = qbload<T>(rB, (r < RW && r < n && c0 + j < nrhs) ? (j * (int)ldb + r) * ES : kQOob);
+        *reinterpret_cast<T *>(lds + r * RS + j * ES) =
+            qbload<T>(rB, (r < RW && r < n && c0 + j < nrhs) ? (j * (int)ldb + r) * ES : kQOob);
     }
     lds_barrier();
-    // panels past the last (at most two, to fill the rotation) store
-    // nothing: their rows are past n
-    for (int p = 0; p < np; p += 3) {
-        panel(p, S0, S2);
-        panel(p + 1, S1, S0);
-        panel(p + 2, S2, S1);
+    // six panels per trip (three operand sets x two window buffers); panels
+    // past the last store nothing: their rows are past n
+    const std::integral_constant<int, 0> b0;
+    const std::integral_constant<int, 1> b1;
+    for (int p = 0; p < np; p += 6) {
+        panel(p, b0, S0, S2);
+        panel(p + 1, b1, S1, S0);
+        panel(p + 2, b0, S2, S1);
+        panel(p + 3, b1, S0, S2);
+        panel(p + 4, b0, S1, S0);
+        panel(p + 5, b1, S2, S1);
     }
 }
 
@@ -381,14 +411,14 @@ __global__ __launch_bounds__(64 * NW) void trs_bwd(T *B, i64 ldb, int nrhs, TrsL
     using v4 = typename Mf<T>::v4;
     constexpr int SW = TStride<T>::value;
     constexpr int NS = PB / 4;
-    constexpr int NP = PB / 16;  // (MT: U tiles per wave)
+    constexpr int NP = PB / 16;  // (MT: V tiles per wave)
     constexpr int ES = (int)sizeof(T);
+    constexpr int RS = SW * ES;
     using Ops = PanelOps<T, NS, MT>;
     extern __shared__ __align__(16) unsigned char tb_raw[];
     const int n = L.n, KTU = L.KTU, np = L.np;
     const int top = 16 * KTU;  // rows above a panel that it updates (>= kv)
-    T *Yr = reinterpret_cast<T *>(tb_raw);  // ring [RB][SW], row i in slot i & (RB-1)
-    T *Xb = Yr + RB * SW;                   // [PB][SW]
+    unsigned char *Yr = tb_raw;  // ring [RB][SW], row i in slot i & (RB-1)
     const int msk = RB - 1;
     const int tid = threadIdx.x, w = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
     const int jc = lane & 15, kq = lane >> 4;
@@ -402,6 +432,15 @@ __global__ __launch_bounds__(64 * NW) void trs_bwd(T *B, i64 ldb, int nrhs, TrsL
 #pragma unroll
     for (int m = 0; m < MT; ++m)
         offB[m] = (int)((L.ublk_off + ((size_t)min(w + m * NW, KTU - 1) * 64 + lane) * NS) * ES);
+    // lane parts of the ring addresses: B operand rows 4s+kq, result rows drow(kq, v)
+    const int yb = kq * RS + jc * ES;
+    int zb[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) zb[v] = drow<T>(kq, v) * RS + jc * ES;
+    int xsto[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+        xsto[v] = (w < NP && colok) ? (jc * (int)ldb + 16 * w + drow<T>(kq, v)) * ES : kQOob;
 
     auto issue = [&](int p, Ops &o) {
         const int j0 = p * PB;
@@ -414,51 +453,47 @@ __global__ __launch_bounds__(64 * NW) void trs_bwd(T *B, i64 ldb, int nrhs, TrsL
         o.fresh = qbload<T>(rB, (fr < PB && frow >= 0 && c0 + fj < nrhs) ? (fj * (int)ldb + frow) * ES : kQOob);
     };
 
-    // panel index q counts from the last panel (q = 0 is panel np-1)
+    // panel index q counts from the last panel (q = 0 is panel np-1).  Ring
+    // slots of negative rows (first panels) take throwaway updates: they are
+    // never read, as RB >= top + 2 PB keeps them clear of live rows.
     auto panel = [&](int q, const Ops &u, Ops &ld) {
         const int p = np - 1 - q;
         const int j0 = p * PB;
-        // X1 = U11^{-1} Y[j0:j0+PB]
-        v4 x = {T(0), T(0), T(0), T(0)};
-        if (w < NP) {
-            v4 x1 = {T(0), T(0), T(0), T(0)};
-#pragma unroll
-            for (int s = 0; s < NS; s += 2) {
-                x = Mf<T>::mma(u.a[s], Yr[((j0 + 4 * s + kq) & msk) * SW + jc], x);
-                x1 = Mf<T>::mma(u.a[s + 1], Yr[((j0 + 4 * s + 4 + kq) & msk) * SW + jc], x1);
-            }
-            x += x1;
-#pragma unroll
-            for (int v = 0; v < 4; ++v) Xb[(16 * w + drow<T>(kq, v)) * SW + jc] = x[v];
-        }
-        lds_barrier();
         issue(max(p - 2, 0), ld);
-        // Y[j0-top : j0] -= U[j0-top : j0, j0 : j0+PB] X1
+        // Y[j0:j0+PB] as B operands (j0 & msk is a multiple of PB: no wrap inside)
+        const unsigned char *yrow = Yr + (j0 & msk) * RS + yb;
+        T y0[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) y0[s] = *reinterpret_cast<const T *>(yrow + 4 * s * RS);
+        // Y[j0-top : j0] += (-V) Y[j0:j0+PB]
 #pragma unroll
         for (int m = 0; m < MT; ++m) {
             const int t = w + m * NW;
             if (t < KTU) {
-                v4 z = {T(0), T(0), T(0), T(0)}, z1 = {T(0), T(0), T(0), T(0)};
+                unsigned char *zrow = Yr + ((j0 - top + 16 * t) & msk) * RS;
+                v4 z;
 #pragma unroll
-                for (int s = 0; s < NS; s += 2) {
-                    z = Mf<T>::mma(u.b[m][s], Xb[(4 * s + kq) * SW + jc], z);
-                    z1 = Mf<T>::mma(u.b[m][s + 1], Xb[(4 * s + 4 + kq) * SW + jc], z1);
-                }
-                z += z1;
+                for (int v = 0; v < 4; ++v) z[v] = *reinterpret_cast<const T *>(zrow + zb[v]);
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int row = j0 - top + 16 * t + drow<T>(kq, v);
-                    if (row >= 0) Yr[(row & msk) * SW + jc] -= z[v];
-                }
+                for (int s = 0; s < NS; ++s) z = Mf<T>::mma(u.b[m][s], y0[s], z);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) *reinterpret_cast<T *>(zrow + zb[v]) = z[v];
             }
         }
         const int frow = j0 - top - PB + fr;
-        if (fr < PB && frow >= 0) Yr[(frow & msk) * SW + fj] = u.fresh;
+        if (fr < PB && frow >= 0) *reinterpret_cast<T *>(Yr + (frow & msk) * RS + fj * ES) = u.fresh;
+        // X1 = U11^{-1} Y[j0:j0+PB], to B
+        v4 x = {T(0), T(0), T(0), T(0)};
+        if (w < NP) {
+#pragma unroll
+            for (int s = 0; s < NS; ++s) x = Mf<T>::mma(u.a[s], y0[s], x);
+        }
+        const bool full = j0 >= 0 && j0 + PB <= n;
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int r = 16 * w + drow<T>(kq, v);
-            const bool ok = w < NP && colok && j0 + r >= 0 && j0 + r < n;
-            qbstore<T>(rB, ok ? (jc * (int)ldb + j0 + r) * ES : kQOob, x[v]);
+            const int off = (full || (j0 + r >= 0 && j0 + r < n)) && xsto[v] != kQOob ? xsto[v] + j0 * ES : kQOob;
+            qbstore<T>(rB, off, x[v]);
         }
         lds_barrier();
     };
@@ -471,12 +506,12 @@ __global__ __launch_bounds__(64 * NW) void trs_bwd(T *B, i64 ldb, int nrhs, TrsL
     for (int e = tid; e < (top + PB) * 16; e += 64 * NW) {
         const int rr = e >> 4, j = e & 15;
         const int row = jl - top + rr;
-        Yr[(row & msk) * SW + j] =
+        *reinterpret_cast<T *>(Yr + (row & msk) * RS + j * ES) =
             qbload<T>(rB, (row >= 0 && row < n && c0 + j < nrhs) ? (j * (int)ldb + row) * ES : kQOob);
     }
     lds_barrier();
     // panels before the first (at most two) touch only negative rows: their
-    // stores are dropped and their ring updates are skipped
+    // stores are dropped and their ring updates land in throwaway slots
     for (int q = 0; q < np; q += 3) {
         panel(q, S0, S2);
         panel(q + 1, S1, S0);
@@ -502,12 +537,12 @@ size_t trs_blocked_bytes(i64 n, i64 kl, i64 ku) {
     const i64 mtf = sizeof(T) == 4 ? 3 : 1, mtb = sizeof(T) == 4 ? 4 : 2;
     if (kl > 16 * mtf * kTrsNW || kl + ku > 16 * mtb * kTrsNW) return 0;
     const Lt L = Lt::make(n, kl, ku);
-    const int RWP = 16 * ((L.RW + 15) / 16);
-    const size_t fwd_lds = (size_t)(2 * RWP + kTrsPB) * SW * sizeof(T);
+    const int RWP = 16 * ((L.RW + 15) / 16) + 1;
+    const size_t fwd_lds = (size_t)(2 * RWP) * SW * sizeof(T);
     int RB = 1;
     while (RB < 16 * L.KTU + 2 * kTrsPB) RB <<= 1;
-    const size_t bwd_lds = (size_t)(RB + kTrsPB) * SW * sizeof(T);
-    const size_t prep_lds = ((size_t)L.RW * (kTrsPB + 1) + 2 * kTrsPB * (kTrsPB + 1)) * sizeof(T) + 2 * L.RW * sizeof(int);
+    const size_t bwd_lds = (size_t)RB * SW * sizeof(T);
+    const size_t prep_lds = ((size_t)(L.RW + 2 * kTrsPB + 16 * L.KTU) * (kTrsPB + 1)) * sizeof(T) + 2 * L.RW * sizeof(int);
     if (fwd_lds > 160 * 1024 || bwd_lds > 160 * 1024 || prep_lds > 160 * 1024) return 0;
     const size_t recs = (size_t)L.np * L.per_panel * sizeof(T);
     const size_t idxs = (size_t)L.np * L.idx_u16 * 2;
@@ -522,7 +557,7 @@ int trs_blocked_prepare(i64 n, i64 kl, i64 ku, const T *ab, i64 ldab, const i64 
     const size_t recs = (size_t)L.np * L.per_panel * sizeof(T);
     T *rec = (T *)work;
     unsigned short *idx = (unsigned short *)((char *)work + ((recs + 255) & ~(size_t)255));
-    const size_t lds = ((size_t)L.RW * (kTrsPB + 1) + 2 * kTrsPB * (kTrsPB + 1)) * sizeof(T) + 2 * L.RW * sizeof(int);
+    const size_t lds = ((size_t)(L.RW + 2 * kTrsPB + 16 * L.KTU) * (kTrsPB + 1)) * sizeof(T) + 2 * L.RW * sizeof(int);
     auto kern = trs_prep<T, kTrsPB>;
     if (lds > 64 * 1024)
         BH_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -546,8 +581,8 @@ int trs_blocked_solve(i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 ldab, co
     const T *rec = (const T *)work;
     const unsigned short *idx = (const unsigned short *)((const char *)work + ((recs + 255) & ~(size_t)255));
     const dim3 grid((unsigned)((nrhs + 15) / 16));
-    const int RWP = 16 * ((L.RW + 15) / 16);
-    const size_t fl = (size_t)(2 * RWP + kTrsPB) * SW * sizeof(T);
+    const int RWP = 16 * ((L.RW + 15) / 16) + 1;
+    const size_t fl = (size_t)(2 * RWP) * SW * sizeof(T);
     const int mtf = (L.KT + kTrsNW - 1) / kTrsNW;
     void (*kf)(T *, i64, int, Lt, const unsigned short *, const T *) = nullptr;
     if constexpr (sizeof(T) == 4) {
@@ -564,7 +599,7 @@ int trs_blocked_solve(i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 ldab, co
     BH_LAUNCH_CHECK();
     int RB = 1;
     while (RB < 16 * L.KTU + 2 * kTrsPB) RB <<= 1;
-    const size_t bl = (size_t)(RB + kTrsPB) * SW * sizeof(T);
+    const size_t bl = (size_t)RB * SW * sizeof(T);
     const int mtb = (L.KTU + kTrsNW - 1) / kTrsNW;
     void (*kb)(T *, i64, int, Lt, const T *, int) = nullptr;
     if constexpr (sizeof(T) == 4) {

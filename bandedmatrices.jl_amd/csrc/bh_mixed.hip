@@ -49,8 +49,11 @@ __device__ __forceinline__ void atomic_max_nonneg(double *addr, double v) {
     atomicMax(reinterpret_cast<unsigned long long *>(addr), (unsigned long long)__double_as_longlong(v));
 }
 
-// out[c] = max_r |v[r,c]| (out must start at 0)
-__global__ void colmax_kernel(i64 rows, i64 cols, const double *__restrict__ v, i64 ld, double *out) {
+// out[c] = max_r |v[r,c]| (out must start at 0); one atomic per workgroup
+// (per-wave atomics on one address serialise)
+__global__ void __launch_bounds__(256) colmax_kernel(i64 rows, i64 cols, const double *__restrict__ v, i64 ld,
+                                                     double *out) {
+    __shared__ double wm[4];
     const i64 c = blockIdx.y;
     double m = 0.0;
     for (i64 r = (i64)blockIdx.x * blockDim.x + threadIdx.x; r < rows; r += (i64)gridDim.x * blockDim.x) {
@@ -62,11 +65,18 @@ __global__ void colmax_kernel(i64 rows, i64 cols, const double *__restrict__ v, 
         const double o = __shfl_xor(m, off, 64);
         m = (o > m || o != o) ? o : m;
     }
-    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(&out[c], m);
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 4; ++k) m = (wm[k] > m || wm[k] != wm[k]) ? wm[k] : m;
+        atomic_max_nonneg(&out[c], m);
+    }
 }
 
-// out = max_i sum_j |A(i,j)| over the band (||A||_inf)
-__global__ void band_inf_norm_kernel(i64 n, i64 kl, i64 ku, const double *__restrict__ A, i64 lda, double *out) {
+// out = max_i sum_j |A(i,j)| over the band (||A||_inf); one atomic per workgroup
+__global__ void __launch_bounds__(256) band_inf_norm_kernel(i64 n, i64 kl, i64 ku, const double *__restrict__ A,
+                                                            i64 lda, double *out) {
+    __shared__ double wm[4];
     const i64 i = (i64)blockIdx.x * blockDim.x + threadIdx.x;
     double s = 0.0;
     if (i < n) {
@@ -77,7 +87,12 @@ __global__ void band_inf_norm_kernel(i64 n, i64 kl, i64 ku, const double *__rest
         const double o = __shfl_xor(s, off, 64);
         s = (o > s || o != o) ? o : s;
     }
-    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(out, s);
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < 4; ++k) s = (wm[k] > s || wm[k] != wm[k]) ? wm[k] : s;
+        atomic_max_nonneg(out, s);
+    }
 }
 
 __global__ void fill_zero_kernel(double *p, i64 cnt) {
@@ -298,9 +313,9 @@ static int dsgbsv_impl(i64 n, i64 kl, i64 ku, i64 nrhs, const double *A, i64 lda
             st = bh_dgbmv_multi('N', n, n, kl, ku, nrhs, -1.0, A, lda, X, ldx, 1.0, w.r, n);
             if (st) return st;
             hipLaunchKernelGGL(fill_zero_kernel, grid1(2 * nrhs), dim3(256), 0, s, w.norms + 1, 2 * nrhs);
-            hipLaunchKernelGGL(colmax_kernel, dim3((unsigned)imin(cdiv(n, 256), 256), (unsigned)nrhs), dim3(256), 0, s,
+            hipLaunchKernelGGL(colmax_kernel, dim3((unsigned)imin(cdiv(n, 2048), 64), (unsigned)nrhs), dim3(256), 0, s,
                                n, nrhs, X, ldx, w.norms + 1);
-            hipLaunchKernelGGL(colmax_kernel, dim3((unsigned)imin(cdiv(n, 256), 256), (unsigned)nrhs), dim3(256), 0, s,
+            hipLaunchKernelGGL(colmax_kernel, dim3((unsigned)imin(cdiv(n, 2048), 64), (unsigned)nrhs), dim3(256), 0, s,
                                n, nrhs, w.r, n, w.norms + 1 + nrhs);
             BH_LAUNCH_CHECK();
             BH_HIP(hipMemcpyAsync(nh.data(), w.norms, (1 + 2 * nrhs) * sizeof(double), hipMemcpyDeviceToHost, s));

@@ -63,7 +63,8 @@ def test_gbmv_t(bh, oracle, dtype, m, n, kl, ku, alpha, beta):
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("m,n,kl,ku,nrhs", [(1000, 1000, 8, 8, 64), (4099, 4099, 4, 3, 5), (777, 900, 20, 3, 16),
-                                            (2048, 2048, 128, 128, 16), (300, 300, 0, 0, 3)])
+                                            (2048, 2048, 128, 128, 16), (300, 300, 0, 0, 3),
+                                            (1500, 1700, 140, 90, 20), (700, 500, 90, 200, 2)])
 def test_gbmv_multi_bitexact(bh, oracle, dtype, m, n, kl, ku, nrhs):
     rng = np.random.default_rng(nrhs + m)
     A = band(rng, kl, ku, n, dtype, dist="n", nan_corners=True, m=m)
