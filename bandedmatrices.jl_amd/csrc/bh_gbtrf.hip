@@ -148,6 +148,8 @@ int mfma_gbtrf_launch(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, 
 // factors agree with it to rounding.  Otherwise the xGBTF2-exact kernels:
 // the one-wave kernel for kl <= 16, the register-window blocked kernel for
 // kl >= 16 where that did not apply, the per-step register kernel otherwise.
+constexpr i64 kLongLu = 16384;  // systems at least this long take the MFMA factorisation (kl, ku >= 8)
+
 static int lu_kernel_override() {
     const char *e = getenv("BANDEDHIP_LU_KERNEL");
     if (!e) return 0;
@@ -179,7 +181,13 @@ static int gbtrf_impl(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, 
     // otherwise the barrier-stepped kernel working in global memory
     const int ov = lu_kernel_override();
     int st = -1;
-    if (ov == 5 || (ov == 0 && ku >= 65 && kl >= 32))
+    // The MFMA-blocked factorisation (pivots equal xGBTF2's, factors to
+    // rounding): where the reference itself runs blocked xGBTRF (ku >= 65,
+    // kl >= 32), and for long systems with kl, ku >= 8, where the one-wave
+    // chain of the narrow kernels is the longer path (n = 262144, (16,16):
+    // 142 ms against 278 ms).  Shorter narrow systems keep the bit-exact
+    // xGBTF2 kernels.
+    if (ov == 5 || (ov == 0 && ((ku >= 65 && kl >= 32) || (kl >= 8 && ku >= 8 && imin(m, n) >= kLongLu))))
         st = mfma_gbtrf_launch<T>(m, n, kl, ku, ab, ldab, ipiv, dinfo, stream());
     // The one-wave lane kernel exists for kl <= 8 and kl = 16; above 16 the
     // blocked kernel beats the generic one-wave kernel (at n = 65536, (24,24):

@@ -117,10 +117,14 @@ int bh_dsgbpad(bh_int m, bh_int n, bh_int kl, bh_int ku, const double *A, bh_int
 /* Partial-pivot band LU in place on the padded array AB (ldab >= 2kl+ku+1,
  * diagonal at row kl+ku), pivots into ipiv (device, 1-based, min(m,n)
  * entries), LAPACK info into *info_host (0 ok, j>0 zero pivot in column j).
- * Factors and pivots equal LAPACK dgbtf2 bit for bit (bands with kl >= 16 use
- * a blocked factorisation that keeps dgbtf2's per-entry operation order; on
- * inputs containing -0.0 or non-finite values it may differ in zero signs
- * and in how far NaN spreads).
+ * Pivots equal LAPACK dgbtf2's.  Factors equal dgbtf2's bit for bit except
+ * where an MFMA-blocked factorisation runs: ku >= 65 with kl >= 32 (where
+ * the reference's LAPACK.gbtrf! itself runs blocked dgbtrf) and
+ * min(m, n) >= 16384 with kl, ku >= 8; there the factors agree to rounding
+ * (the updates of a 16-column panel are summed at once).  Bands with kl >= 16
+ * below those bounds use a blocked factorisation that keeps dgbtf2's
+ * per-entry operation order (on inputs containing -0.0 or non-finite values
+ * it may differ in zero signs and in how far NaN spreads).
  * Replaces LAPACK.gbtrf! in lu! (src/banded/BandedLU.jl:90-103). */
 int bh_dgbtrf(bh_int m, bh_int n, bh_int kl, bh_int ku, double *AB, bh_int ldab,
               bh_int *ipiv, bh_int *info_host);

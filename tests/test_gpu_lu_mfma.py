@@ -1,5 +1,6 @@
-"""GPU parity: the MFMA-blocked band LU (ku >= 65, kl >= 32), the regime in
-which the reference's LAPACK.gbtrf! runs blocked xGBTRF.
+"""GPU parity: the MFMA-blocked band LU -- for ku >= 65, kl >= 32 (the
+regime in which the reference's LAPACK.gbtrf! runs blocked xGBTRF) and for
+long systems (min(m, n) >= 16384) with kl, ku >= 8.
 
 Pivots (ipiv) and info must equal the oracle's (xGBTF2 restated, which picks
 the same pivots as xGBTRF); factors must agree to rounding (FP64: 1e-12 of
@@ -27,6 +28,10 @@ CASES = [
     (300, 400, 40, 70, "n", 0.0),
     (400, 300, 40, 70, "n", 0.0),
     (40, 40, 32, 65, "n", 0.0),
+    (16384, 16384, 8, 8, "n", 0.0),      # long narrow systems
+    (20000, 20000, 16, 16, "n", 0.0),
+    (20000, 20000, 16, 16, "n", 64.0),
+    (17000, 16500, 12, 9, "u", 0.0),
 ]
 
 
