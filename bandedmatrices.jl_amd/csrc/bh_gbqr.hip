@@ -23,8 +23,17 @@
 //     entering at the right are prefetched PF steps ahead.
 //   * gbqr_apply_wave: one wave per right-hand side; lanes hold b over the
 //     reflector's rows; w = b_k + sum x_i b_i is a DPP tree sum.
-//   * gbqrf_global: any band, one workgroup, reflector by thread 0 and one
-//     thread per column for the applies, barriers between (slow path).
+//   * gbqrf_wide: bands beyond the wave kernel's window (l+1 > 32 or
+//     u'+1 > 64, l+1 <= 512): one 1024-thread workgroup; per column wave 0
+//     forms the reflector (norm as a wave-parallel sum), then every wave
+//     applies it to every 16th later column of the band window, 64 lanes on
+//     a column's rows with a DPP-reduced dot.  The sums run in a different
+//     order than the reference's sequential loops: factors agree to rounding.
+//   * gbqr_apply_wide: Q'B / QB for l+1 > 64: one wave per right-hand side,
+//     the right-hand side resident in LDS, the same wave-parallel dot per
+//     reflector (m up to kWideApplyRows).
+//   * gbqrf_global / gbqr_apply_serial: any band, sequential sums in the
+//     reference's order (fallback beyond those limits).
 // The QR solve (bandedqr.jl:178-206) is Q'B followed by the upper-triangular
 // band solve of R (tribanded.jl:75-83 -> tbsv), which is the back
 // substitution of the band LU solve with no L part.
@@ -165,6 +174,101 @@ __global__ void __launch_bounds__(64) gbqrf_wave(int m, int n, int l, int u, T *
     }
 }
 
+// ---------------------------------------------------------------- factor, wide bands
+// RR rows per lane (l+1 <= 64 RR); 16 waves.
+template <typename T, int RR>
+__global__ void __launch_bounds__(1024) gbqrf_wide(int m, int n, int l, int u, T *D, i64 ld, T *tau) {
+    __shared__ T sv[64 * RR];  // the current reflector v (v_0 = 1)
+    __shared__ T stk;
+    const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int mn = m < n ? m : n;
+    const int ncols = (m - 1 < n) ? m - 1 : n;
+    for (int k = tid; k < mn; k += blockDim.x) tau[k] = T(0);
+    __syncthreads();
+    for (int k = 0; k < ncols; ++k) {
+        const int len = (l + 1 < m - k) ? l + 1 : m - k;
+        const int jmax = (u < n - 1 - k) ? u : n - 1 - k;
+        if (w == 0) {
+            T *x = D + (i64)k * ld + u;
+            T xv[RR];
+            T ss = T(0);
+#pragma unroll
+            for (int r = 0; r < RR; ++r) {
+                const int i = lane + 64 * r;
+                xv[r] = i < len ? x[i] : T(0);
+                ss = ss + xv[r] * xv[r];
+            }
+            const T nrm2 = wave_sum_dpp(ss);
+            T tk = T(0);
+            if (nrm2 != T(0)) {
+                const T nrm = qsqrt(nrm2);
+                const T x0 = lane_read(xv[0], 0);
+                const T nu = qcopysign(nrm, x0);
+                const T xi1 = x0 + nu;
+#pragma unroll
+                for (int r = 0; r < RR; ++r) {
+                    const int i = lane + 64 * r;
+                    xv[r] = (i == 0) ? -nu : xv[r] / xi1;
+                }
+                tk = xi1 / nu;
+            }
+#pragma unroll
+            for (int r = 0; r < RR; ++r) {
+                const int i = lane + 64 * r;
+                if (i < len) {
+                    x[i] = xv[r];
+                    sv[i] = (i == 0) ? T(1) : xv[r];
+                }
+            }
+            if (lane == 0) {
+                tau[k] = tk;
+                stk = tk;
+            }
+        }
+        __syncthreads();
+        const T tk = stk;
+        if (tk != T(0)) {
+            T v[RR];
+#pragma unroll
+            for (int r = 0; r < RR; ++r) {
+                const int i = lane + 64 * r;
+                v[r] = i < len ? sv[i] : T(0);
+            }
+            // y <- y - v (tk v'y) for columns k+1 .. k+jmax, two columns in flight
+            for (int jj = 1 + w; jj <= jmax; jj += 32) {
+                const int jj2 = jj + 16;
+                const bool two = jj2 <= jmax;  // wave-uniform
+                T *y = D + (i64)(k + jj) * ld + (u - jj);
+                T *y2 = D + (i64)(k + jj2) * ld + (u - jj2);
+                T a[RR], b[RR];
+                T s1 = T(0), s2 = T(0);
+#pragma unroll
+                for (int r = 0; r < RR; ++r) {
+                    const int i = lane + 64 * r;
+                    a[r] = i < len ? y[i] : T(0);
+                    b[r] = (two && i < len) ? y2[i] : T(0);
+                }
+#pragma unroll
+                for (int r = 0; r < RR; ++r) {
+                    s1 = s1 + v[r] * a[r];
+                    s2 = s2 + v[r] * b[r];
+                }
+                s1 = tk * wave_sum_dpp(s1);
+                s2 = tk * wave_sum_dpp(s2);
+#pragma unroll
+                for (int r = 0; r < RR; ++r) {
+                    const int i = lane + 64 * r;
+                    if (i < len) {
+                        y[i] = a[r] - v[r] * s1;
+                        if (two) y2[i] = b[r] - v[r] * s2;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------- factor, any band
 template <typename T>
 __global__ void __launch_bounds__(256) gbqrf_global(int m, int n, int l, int u, T *D, i64 ld, T *tau) {
@@ -278,6 +382,57 @@ __global__ void __launch_bounds__(64) gbqr_apply_wave(bool transpose, int m, int
     }
 }
 
+// Q' B / Q B for l + 1 > 64: one wave per right-hand side, b resident in LDS
+// (m <= kWideApplyRows), reflector v of the next step loaded during this one.
+constexpr int kWideApplyRows = 12288;
+template <typename T, int RR>
+__global__ void __launch_bounds__(64) gbqr_apply_wide(bool transpose, int m, int l, int u, const T *D, i64 ld,
+                                                      const T *tau, int kmax, T *B, i64 ldb) {
+    extern __shared__ __align__(16) unsigned char qa_raw[];
+    T *bs = reinterpret_cast<T *>(qa_raw);
+    const int lane = threadIdx.x;
+    T *b = B + (i64)blockIdx.x * ldb;
+    for (int i = lane; i < m; i += 64) bs[i] = b[i];
+    auto load_v = [&](int t, T (&v)[RR], T &tk) {
+        const int k = transpose ? t : kmax - 1 - t;
+        const int len = (l + 1 < m - k) ? l + 1 : m - k;
+        const T *x = D + (i64)k * ld + u;
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
+            const int i = lane + 64 * r;
+            v[r] = (i == 0) ? T(1) : (i < len ? x[i] : T(0));
+        }
+        tk = tau[k];
+    };
+    T v[RR], vn[RR], tk, tkn;
+    load_v(0, v, tk);
+    for (int t = 0; t < kmax; ++t) {
+        if (t + 1 < kmax) load_v(t + 1, vn, tkn);
+        const int k = transpose ? t : kmax - 1 - t;
+        const int len = (l + 1 < m - k) ? l + 1 : m - k;
+        T y[RR];
+        T s = T(0);
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
+            const int i = lane + 64 * r;
+            y[r] = i < len ? bs[k + i] : T(0);
+            s = s + v[r] * y[r];
+        }
+        s = tk * wave_sum_dpp(s);
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
+            const int i = lane + 64 * r;
+            if (i < len) bs[k + i] = y[r] - v[r] * s;
+        }
+        if (t + 1 < kmax) {
+#pragma unroll
+            for (int r = 0; r < RR; ++r) v[r] = vn[r];
+            tk = tkn;
+        }
+    }
+    for (int i = lane; i < m; i += 64) b[i] = bs[i];
+}
+
 // Q' B / Q B with reflectors k < kmax applied in sequence, for bands l + 1 > 64
 // (thread 0 per right-hand side, slow path).
 template <typename T>
@@ -362,6 +517,17 @@ int gbqrf_impl(i64 m, i64 n, i64 l, i64 u, T *D, i64 ldd, T *tau) {
     BH_QR_CASE(16)
     BH_QR_CASE(32)
 #undef BH_QR_CASE
+#define BH_QR_WIDE(RRv)                                                                       \
+    if (l + 1 <= 64 * RRv) {                                                                  \
+        hipLaunchKernelGGL((gbqrf_wide<T, RRv>), dim3(1), dim3(1024), 0, st, mi, ni, li, ui, D, ldd, tau); \
+        BH_LAUNCH_CHECK();                                                                    \
+        return BH_OK;                                                                         \
+    }
+    BH_QR_WIDE(1)
+    BH_QR_WIDE(2)
+    BH_QR_WIDE(4)
+    BH_QR_WIDE(8)
+#undef BH_QR_WIDE
     hipLaunchKernelGGL(gbqrf_global<T>, dim3(1), dim3(256), 0, st, mi, ni, li, ui, D, ldd, tau);
     BH_LAUNCH_CHECK();
     return BH_OK;
@@ -385,6 +551,26 @@ int gbqrm_impl(char trans, i64 m, i64 n, i64 l, i64 u, const T *D, i64 ldd, cons
     if (l + 1 <= 64) {
         hipLaunchKernelGGL(gbqr_apply_wave<T>, dim3((unsigned)nrhs), dim3(64), 0, st, tr, (int)m, (int)n, (int)l,
                            (int)u, D, ldd, tau, kmax, B, ldb);
+    } else if (m <= kWideApplyRows && l + 1 <= 64 * 16) {
+        const size_t lds = (size_t)m * sizeof(T);
+        const int rr = (int)cdiv(l + 1, 64);
+        auto launch = [&](auto kern) -> int {
+            if (lds > 64 * 1024)
+                BH_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL(kern, dim3((unsigned)nrhs), dim3(64), lds, st, tr, (int)m, (int)l, (int)u, D, ldd, tau,
+                               kmax, B, ldb);
+            return BH_OK;
+        };
+        int ls;
+        if (rr <= 2)
+            ls = launch(gbqr_apply_wide<T, 2>);
+        else if (rr <= 4)
+            ls = launch(gbqr_apply_wide<T, 4>);
+        else if (rr <= 8)
+            ls = launch(gbqr_apply_wide<T, 8>);
+        else
+            ls = launch(gbqr_apply_wide<T, 16>);
+        if (ls) return ls;
     } else {
         hipLaunchKernelGGL(gbqr_apply_serial<T>, dim3((unsigned)cdiv(nrhs, 64)), dim3(64), 0, st, tr, (int)m, (int)n,
                            (int)l, (int)u, D, ldd, tau, kmax, (int)nrhs, B, ldb);

@@ -40,24 +40,55 @@ def _padded(oracle, A, m, n, l, u):
     return oracle.pad_for_lu(A, m, n, l, u)
 
 
-@pytest.mark.parametrize("m,n,l,u", QR_CASES)
-def test_gbqrf_bits(bh, oracle, m, n, l, u):
+def _wave_path(l, u):
+    """The one-wave factorisation (sequential sums in the reference's order)
+    covers l+1 <= 32 and padded u' = l+u with u'+1 <= 64."""
+    return l + 1 <= 32 and l + u + 1 <= 64
+
+
+WIDE_CASES = [(500, 500, 100, 100), (400, 300, 70, 30), (300, 400, 70, 30), (600, 600, 300, 50), (257, 257, 64, 1)]
+
+
+@pytest.mark.parametrize("m,n,l,u", QR_CASES + WIDE_CASES)
+def test_gbqrf_factors(bh, oracle, m, n, l, u):
+    """Bit-exact on the one-wave path.  On the wide path (wave-parallel norms
+    and dot products) the reflectors differ in rounding, which the later
+    columns of an ill-conditioned random band amplify: there Q R must equal A
+    to 1e-13 (relative, max norm) and |diag R| the oracle's to 1e-10."""
     rng = np.random.default_rng(m * 7 + n + l * 31 + u)
     A = _make(rng, m, n, l, u)
     F = bh.qr(bh.BandedMatrix.from_host(A, m, l, u))
     D, tau = oracle.gbqrf(m, n, l, l + u, _padded(oracle, A, m, n, l, u))
     g = F.factors.host_data()
-    assert bits_equal(g, D), np.nanmax(np.abs(g - D))
-    assert bits_equal(F.tau.numpy()[: min(m, n)], tau)
+    t = F.tau.numpy()[: min(m, n)]
+    if _wave_path(l, u):
+        assert bits_equal(g, D), np.nanmax(np.abs(g - D))
+        assert bits_equal(t, tau)
+    else:
+        # Q R = A to backward-error accuracy, with Q applied from the GPU's
+        # own reflectors by the oracle; R's |diagonal| as the oracle's
+        up = l + u
+        R = np.zeros((m, n))
+        for j in range(n):
+            for i in range(max(0, j - up), min(j, m - 1) + 1):
+                R[i, j] = g[up + i - j, j]
+        QR = oracle.gbqr_qmul(m, n, l, up, g, t, R)
+        Ad = oracle.band_to_dense(A, m, n, l, u)
+        err = np.abs(QR - Ad).max() / np.abs(Ad).max()
+        assert err <= 1e-13, err
+        Ro = np.array([D[up, j] for j in range(min(m, n))])
+        assert np.abs(np.abs(np.diag(R)[: min(m, n)]) - np.abs(Ro)).max() <= 1e-10 * np.abs(Ro).max()
 
 
-@pytest.mark.parametrize("m,n,l,u", [c for c in QR_CASES if c[0] <= 2000])
+@pytest.mark.parametrize("m,n,l,u", [c for c in QR_CASES + WIDE_CASES if c[0] <= 2000])
 @pytest.mark.parametrize("nrhs", [1, 3])
 def test_q_apply(bh, oracle, m, n, l, u, nrhs):
     rng = np.random.default_rng(m + 3 * n + 5 * l + u + nrhs)
     A = _make(rng, m, n, l, u)
     F = bh.qr(bh.BandedMatrix.from_host(A, m, l, u))
-    D, tau = oracle.gbqrf(m, n, l, l + u, _padded(oracle, A, m, n, l, u))
+    # the oracle applies the GPU's own reflectors (on the wide path they differ
+    # from the oracle's factorisation in rounding)
+    D, tau = F.factors.host_data(), F.tau.numpy()[: min(m, n)]
     B = np.asfortranarray(rng.standard_normal((m, nrhs)))
     for adj, ref in ((True, oracle.gbqr_qtmul), (False, oracle.gbqr_qmul)):
         Q = F.Q.T if adj else F.Q
@@ -71,7 +102,7 @@ def test_q_apply(bh, oracle, m, n, l, u, nrhs):
     assert np.abs(dB.numpy() - B).max() <= 1e-13 * max(1.0, np.abs(B).max())
 
 
-@pytest.mark.parametrize("m,n,l,u", [c for c in QR_CASES if c[0] >= c[1]])
+@pytest.mark.parametrize("m,n,l,u", [c for c in QR_CASES + WIDE_CASES if c[0] >= c[1]])
 def test_qr_solve(bh, oracle, m, n, l, u):
     rng = np.random.default_rng(11 * m + n + l + 2 * u)
     A = _make(rng, m, n, l, u)
@@ -79,7 +110,7 @@ def test_qr_solve(bh, oracle, m, n, l, u):
     B = np.asfortranarray(rng.standard_normal((m, 2)))
     F = bh.qr(bh.BandedMatrix.from_host(A, m, l, u))
     X = bh.qr_ldiv_(F, bh.DeviceArray.from_numpy(B)).numpy()[:n]
-    D, tau = oracle.gbqrf(m, n, l, l + u, _padded(oracle, A, m, n, l, u))
+    D, tau = F.factors.host_data(), F.tau.numpy()[: min(m, n)]  # the GPU's factors, solved by the oracle
     Xo = oracle.gbqrs(m, n, l, l + u, D, tau, B)[:n]
     assert np.abs(X - Xo).max() <= 1e-12 * max(1.0, np.abs(Xo).max())
     # normal equations: A'(A x - b) = 0 (least squares; exact solve when square)
@@ -150,7 +181,7 @@ def test_qr_solve_wide_bands(bh, oracle, n, l, u):
     dB = bh.DeviceArray.from_numpy(B)
     bh.qr_ldiv_(F, dB)
     x = dB.numpy()
-    D, tau = oracle.gbqrf(n, n, l, l + u, _padded(oracle, A, n, n, l, u))
+    D, tau = F.factors.host_data(), F.tau.numpy()[:n]  # the GPU's factors, solved by the oracle
     xref = oracle.gbqrs(n, n, l, l + u, D, tau, B)
     assert np.abs(x - xref).max() <= 1e-10 * (np.abs(xref).max() + 1)
     for c in range(2):
