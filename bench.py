@@ -101,6 +101,26 @@ def cpu_reference_solve(A, B, l, u, threads):
     return x, piv + 1, t1 - t0
 
 
+def backward_error(A, l, u, X, B):
+    """max over columns of ||b - A x||_inf / (||A||_inf ||x||_inf + ||b||_inf), FP64 on the host."""
+    import scipy.linalg.blas as sb
+
+    n = A.shape[1]
+    anrm = 0.0
+    rows = np.zeros(n)
+    for t in range(l + u + 1):  # row sums of |A| from the band rows
+        j = np.arange(n)
+        i = j - u + t
+        ok = (i >= 0) & (i < n)
+        np.add.at(rows, i[ok], np.abs(A[t, ok]))
+    anrm = rows.max()
+    worst = 0.0
+    for c in range(X.shape[1]):
+        r = B[:, c] - sb.dgbmv(n, n, l, u, 1.0, A, X[:, c])
+        worst = max(worst, np.abs(r).max() / (anrm * np.abs(X[:, c]).max() + np.abs(B[:, c]).max()))
+    return float(worst)
+
+
 def gpu_time(bh, fn, reps):
     t = bh.Timer()
     t.start()
@@ -703,6 +723,20 @@ def traffic_probe(bh, name):
         y = bh.DeviceArray((n,), np.float64)
         for _ in range(3):
             bh.mul(y, A, x)
+    elif name == "gbmv64x64" or name == "gbmm64":
+        n = 1 << 20
+        rng = np.random.default_rng(1)
+        A = bh.BandedMatrix.from_host(np.asfortranarray(rng.standard_normal((17, n))), n, 8, 8)
+        if name == "gbmv64x64":
+            X = bh.DeviceArray.from_numpy(np.asfortranarray(rng.standard_normal((n, 64))))
+            Y = bh.DeviceArray((n, 64), np.float64)
+            for _ in range(3):
+                bh.mul(Y, A, X)
+        else:
+            Bm = bh.BandedMatrix.from_host(np.asfortranarray(rng.standard_normal((9, n))), n, 4, 4)
+            Cm = bh.BandedMatrix.zeros((n, n), (12, 12))
+            for _ in range(3):
+                bh.gbmm(A, Bm, Cm)
     elif name == "lu32":
         n, l, u = 65536, 128, 128
         A_h, _ = make_precision_problem(n, l, u, 1)
@@ -710,6 +744,19 @@ def traffic_probe(bh, name):
         ipiv = bh.DeviceArray((n,), np.int64)
         info = C.c_int64()
         _lib.call("bh_sgbtrf", n, n, l, u, ab.data.ptr, ab.data.ld, ipiv.ptr, C.byref(info))
+    elif name == "trs32":
+        n, l, u, nrhs = 65536, 128, 128, 16
+        A_h, B_h = make_precision_problem(n, l, u, nrhs)
+        ab = bh.pad_for_lu(bh.BandedMatrix.from_host(A_h, n, l, u), np.float32)
+        ipiv = bh.DeviceArray((n,), np.int64)
+        info = C.c_int64()
+        _lib.call("bh_sgbtrf", n, n, l, u, ab.data.ptr, ab.data.ld, ipiv.ptr, C.byref(info))
+        nb = C.c_size_t()
+        _lib.call("bh_gbtrs_blocked_workspace", b"s", n, l, u, C.byref(nb))
+        work = bh.DeviceArray((nb.value,), np.uint8)
+        B32 = bh.DeviceArray.from_numpy(np.asfortranarray(B_h.astype(np.float32)))
+        _lib.call("bh_sgbtrs_blocked", n, l, u, nrhs, ab.data.ptr, ab.data.ld, ipiv.ptr, B32.ptr, n, work.ptr,
+                  nb.value)
     bh.sync()
     print(json.dumps({"traffic_probe": name, "copy_bytes": src.nbytes}))
 
@@ -774,7 +821,9 @@ def main():
         log("precision workload, no diagonal shift (pivoting active)")
         P2 = bench_precision(bh, max(1, args.steps // 2), 1, shift=False)
         ns = {"mixed_solves_per_s": 1000.0 / P2["ms_mixed"], "fp64_gpu_solves_per_s": 1000.0 / P2["ms_fp64"],
-              "refinement_iters": P2["iters"]}
+              "refinement_iters": P2["iters"],
+              "backward_error_mixed": backward_error(P2["A_h"], P2["l"], P2["u"], P2["x_mixed"], P2["B_h"]),
+              "backward_error_fp64_gpu": backward_error(P2["A_h"], P2["l"], P2["u"], P2["x_fp64"], P2["B_h"])}
         if not args.skip_cpu:
             xc2, pivc2, tc2 = cpu_reference_solve(P2["A_h"], P2["B_h"], P2["l"], P2["u"], _host_threads())
             fe2 = lambda x: float(np.abs(x - xc2).max() / np.abs(xc2).max())
@@ -809,6 +858,9 @@ def main():
         }
         fe = lambda x: float(np.abs(x - xc).max() / np.abs(xc).max())
         out["forward_error_vs_cpu_fp64"] = {"mixed": fe(P["x_mixed"]), "fp64_gpu": fe(P["x_fp64"])}
+        out["backward_error"] = {"mixed": backward_error(P["A_h"], l, u, P["x_mixed"], P["B_h"]),
+                                 "fp64_gpu": backward_error(P["A_h"], l, u, P["x_fp64"], P["B_h"]),
+                                 "cpu": backward_error(P["A_h"], l, u, xc, P["B_h"])}
         out["pivots_fp64_gpu_equal_cpu"] = f"{int(np.sum(P['ipiv64'] == pivc))}/{n}"
 
     g = extra["gbmv_fp64_8x8_n2p20"]
