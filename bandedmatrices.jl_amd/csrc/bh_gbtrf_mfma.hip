@@ -72,6 +72,11 @@ struct QArgs {
     int R, RT, RP;  // window rows, 16-row tiles, padded rows (16*RT)
 };
 
+template <typename T, int PR>
+struct QCol {
+    typedef T type __attribute__((ext_vector_type(PR)));
+};
+
 template <int K>
 struct QIntC {
     static constexpr int value = K;
@@ -191,6 +196,67 @@ __device__ __forceinline__ int wave_min_int(int v) {
     return (int)(0x7fffffffu - x);
 }
 
+// Pivot of one elimination step, searched over the panel wave's slots: slot r
+// of a lane holds value col[r] at position pos[r]; the candidates are the
+// positions k .. k+km.  The pivot is the largest |a| (NaN below every number,
+// zero above NaN), the smallest position among equals -- xGBTF2's IxAMAX
+// choice on finite data.  Branch-free: one wave maximum of the 32-bit keys
+// (two for the 64-bit keys of FP64), then one of the tied candidates'
+// complemented (position, slot, lane) labels.  Uniform results: the pivot's
+// lane lp, slot rp and position p.
+template <int PR>
+__device__ __forceinline__ void qpivot(const float (&col)[PR], const int (&pos)[PR], int k, int km, int lane,
+                                       int &lp, int &rp, int &p) {
+    unsigned key[PR], kmax = 0;
+    bool cand[PR];
+#pragma unroll
+    for (int r = 0; r < PR; ++r) {
+        cand[r] = (unsigned)(pos[r] - k) <= (unsigned)km;
+        key[r] = cand[r] ? qkey(col[r]) : 0u;
+        kmax = qumax(kmax, key[r]);
+    }
+    const unsigned M = wave_max_key(kmax);
+    unsigned sel = 0;
+#pragma unroll
+    for (int r = 0; r < PR; ++r) {
+        const unsigned lab = ((unsigned)pos[r] << 8) | (unsigned)((r << 6) | lane);
+        sel = qumax(sel, (cand[r] && key[r] == M) ? 0x10000u - lab : 0u);
+    }
+    const unsigned lab = 0x10000u - wave_max_key(sel);
+    lp = (int)(lab & 63u);
+    rp = (int)((lab >> 6) & 3u);
+    p = (int)(lab >> 8);
+}
+template <int PR>
+__device__ __forceinline__ void qpivot(const double (&col)[PR], const int (&pos)[PR], int k, int km, int lane,
+                                       int &lp, int &rp, int &p) {
+    unsigned hi[PR], lo[PR], hmax = 0;
+    bool cand[PR];
+#pragma unroll
+    for (int r = 0; r < PR; ++r) {
+        cand[r] = (unsigned)(pos[r] - k) <= (unsigned)km;
+        const unsigned long long kk = cand[r] ? qkey(col[r]) : 0ull;
+        hi[r] = (unsigned)(kk >> 32);
+        lo[r] = (unsigned)kk;
+        hmax = qumax(hmax, hi[r]);
+    }
+    const unsigned MH = wave_max_key(hmax);
+    unsigned lmax = 0;
+#pragma unroll
+    for (int r = 0; r < PR; ++r) lmax = qumax(lmax, (cand[r] && hi[r] == MH) ? lo[r] : 0u);
+    const unsigned ML = wave_max_key(lmax);
+    unsigned sel = 0;
+#pragma unroll
+    for (int r = 0; r < PR; ++r) {
+        const unsigned lab = ((unsigned)pos[r] << 8) | (unsigned)((r << 6) | lane);
+        sel = qumax(sel, (cand[r] && hi[r] == MH && lo[r] == ML) ? 0x10000u - lab : 0u);
+    }
+    const unsigned lab = 0x10000u - wave_max_key(sel);
+    lp = (int)(lab & 63u);
+    rp = (int)((lab >> 6) & 3u);
+    p = (int)(lab >> 8);
+}
+
 __device__ __forceinline__ int lds_load_relaxed(const int *p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
@@ -271,14 +337,16 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
         for (int P = 0; P < NP; ++P) {
             const int j0 = P * QB, np = qmin(QB, mn - j0), b = P & 1;
             const T *pin = pan + b * RP * QS;
-            T pp[PR][QB];
+            // slot values by column: pp[c][r] is column c of slot r (a short vector
+            // per column, so the pivot slot's value is picked by a uniform index)
+            typename QCol<T, PR>::type pp[QB];
             int pos[PR];
 #pragma unroll
             for (int r = 0; r < PR; ++r) {
                 const int q = lane + 64 * r;
                 pos[r] = (q < R && j0 + q < m) ? q : QBIG;
 #pragma unroll
-                for (int c = 0; c < QB; ++c) pp[r][c] = (q < R) ? pin[q * QS + c] : T(0);
+                for (int c = 0; c < QB; ++c) pp[c][r] = (q < R) ? pin[q * QS + c] : T(0);
             }
             int myp = 0;
             // multipliers of a step go to AB column j through a buffer resource over
@@ -286,118 +354,79 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
             const int ncol = qmin(QB, n - j0);
             const __amdgpu_buffer_rsrc_t rpan = __builtin_amdgcn_make_buffer_rsrc(
                 ab + (i64)j0 * ld, 0, (int)((i64)ncol * ld * (i64)sizeof(T)), kQRsrcWord3);
+            // Each step finds the next step's pivot as soon as the next column has
+            // its update, and updates the remaining columns after that: the
+            // pivot search's wave reductions and the rank-1 update are independent
+            // and interleave in one branch-free block.
+            int lp, rp, p;  // pivot of the current step (uniform)
+            {
+                T c0v[PR];
+#pragma unroll
+                for (int r = 0; r < PR; ++r) c0v[r] = pp[0][r];
+                qpivot<PR>(c0v, pos, 0, qmin(kl, m - 1 - j0), lane, lp, rp, p);
+            }
             QFor<QB>::run([&](auto kc) {
                 constexpr int k = decltype(kc)::value;
                 if (k < np) {
                     const int j = j0 + k;
                     const int km = qmin(kl, m - 1 - j);
-                    // ---- pivot: largest |a| over positions k..k+km, smallest position
-                    // on ties.  Fast path: integer keys from the magnitude's high word,
-                    // one DPP maximum, one ballot per slot; when more than one slot holds
-                    // the maximum key (a tie, equal high words, NaN, an all-zero column)
-                    // the exact search decides.
-                    unsigned key[PR];
-                    unsigned kmax = 0;
-#pragma unroll
-                    for (int r = 0; r < PR; ++r) {
-                        const bool cand = (unsigned)(pos[r] - k) <= (unsigned)km;
-                        const unsigned hw = qhiword(pp[r][k]);
-                        key[r] = (cand && hw != 0xffffffffu) ? hw + 1u : 0u;
-                        kmax = qumax(kmax, key[r]);
-                    }
-                    const unsigned M = wave_max_key(kmax);
-                    unsigned long long tm[PR];
-                    int ntop = 0;
-#pragma unroll
-                    for (int r = 0; r < PR; ++r) {
-                        tm[r] = (unsigned long long)__ballot(key[r] == M);
-                        ntop += __popcll(tm[r]);
-                    }
-                    int lp, p, rp;
-                    if (ntop == 1 && M > 1u) {
-                        rp = 0;
-                        unsigned long long t = tm[0];
-#pragma unroll
-                        for (int r = 1; r < PR; ++r) {
-                            rp = tm[r] ? r : rp;
-                            t = tm[r] ? tm[r] : t;
-                        }
-                        lp = (int)__ffsll((long long)t) - 1;
-                        p = __builtin_amdgcn_readlane(pos[0], lp);
-#pragma unroll
-                        for (int r = 1; r < PR; ++r) {
-                            const int pr_ = __builtin_amdgcn_readlane(pos[r], lp);
-                            p = (rp == r) ? pr_ : p;
-                        }
-                    } else {
-                        // exact: keys over the full value (NaN below every number), the
-                        // smallest position first
-                        unsigned long long bk = 0;
-                        int kp = QBIG, kr = 0;
-#pragma unroll
-                        for (int r = 0; r < PR; ++r) {
-                            const int L = pos[r];
-                            const unsigned long long kk =
-                                (L >= k && L <= k + km) ? (unsigned long long)qkey(pp[r][k]) : 0ull;
-                            const bool better = kk > bk || (kk == bk && L < kp);
-                            bk = better ? kk : bk;
-                            kp = better ? L : kp;
-                            kr = better ? r : kr;
-                        }
-                        const unsigned long long mk = wave_max_key(bk);
-                        const bool tk = (bk != 0ull) && (bk == mk);
-                        const int pmin = wave_min_int(tk ? kp : QBIG);
-                        const unsigned long long t2 = (unsigned long long)__ballot(tk && kp == pmin);
-                        lp = __builtin_amdgcn_readfirstlane((int)__ffsll((long long)t2) - 1);
-                        p = __builtin_amdgcn_readlane(kp, lp);
-                        rp = __builtin_amdgcn_readlane(kr, lp);
-                    }
-                    // ---- the pivot row, columns k .. QB-1 (uniform)
+                    // ---- the pivot row, columns k .. QB-1 (uniform): the pivot slot's
+                    // value picked per column, read from the pivot lane
                     T u[QB];
 #pragma unroll
                     for (int c = 0; c < QB; ++c) u[c] = T(0);
 #pragma unroll
-                    for (int r = 0; r < PR; ++r) {
-                        if (rp == r) {
-#pragma unroll
-                            for (int c = k; c < QB; ++c) u[c] = lane_read(pp[r][c], lp);
-                        }
-                    }
+                    for (int c = k; c < QB; ++c) u[c] = lane_read(pp[c][rp], lp);
                     const T pv = u[k];
                     const bool nz = (pv != T(0));
-                    if (nz) {
-                        ju = qmax(ju, qmin(j + ku + (p - k), n - 1));
-                        // interchange = relabel: the slot at position k takes position p,
-                        // the pivot slot takes position k
+                    // interchange = relabel (nonzero pivot): the slot at position k
+                    // takes position p, the pivot slot takes position k
+                    ju = nz ? qmax(ju, qmin(j + ku + (p - k), n - 1)) : ju;
+                    info = (!nz && info == 0) ? j + 1 : info;
 #pragma unroll
-                        for (int r = 0; r < PR; ++r) {
-                            pos[r] = (pos[r] == k) ? p : pos[r];
-                            pos[r] = (r == rp && lane == lp) ? k : pos[r];
-                        }
-                    } else if (info == 0) {
-                        info = j + 1;
+                    for (int r = 0; r < PR; ++r) {
+                        int L = (pos[r] == k) ? p : pos[r];
+                        L = (r == rp && lane == lp) ? k : L;
+                        pos[r] = nz ? L : pos[r];
                     }
                     const T rcp = nz ? qrcp(pv) : T(1);
                     T *cb = colb + k * (64 * PR);
+                    T le[PR];
 #pragma unroll
                     for (int r = 0; r < PR; ++r) {
                         const int L = pos[r];
                         const bool act = (L > k) && (L <= k + km);
-                        const T raw = pp[r][k];
+                        const T raw = pp[k][r];
                         const T l = nz ? raw * rcp : raw;  // a zero pivot leaves the column as it is
                         const int off = act ? (k * (ldi - 1) + kv + L) * (int)sizeof(T) : kQOob;
                         qbstore<T>(rpan, off, l);
-                        pp[r][k] = act ? l : raw;
-                        const T le = (act && nz) ? l : T(0);
-                        cb[lane + 64 * r] = -le;
-                        // rank-1 update of the panel's later columns; inactive slots take
-                        // multiplier 0 (a signed zero can lose its sign there)
-#pragma unroll
-                        for (int c = k + 1; c < QB; ++c) pp[r][c] = dfma(le, -u[c], pp[r][c]);
+                        pp[k][r] = act ? l : raw;
+                        // inactive slots take multiplier 0 (a signed zero can lose its
+                        // sign in their rank-1 update)
+                        le[r] = (act && nz) ? l : T(0);
+                        cb[lane + 64 * r] = -le[r];
                     }
-                    if (lane == k) myp = p;
+                    const int lp0 = lp, rp0 = rp, p0 = p;
+                    if constexpr (k + 1 < QB) {
+                        // column k+1 first, then the next pivot (its result is unused
+                        // past the panel's last column)
+                        T nx[PR];
+#pragma unroll
+                        for (int r = 0; r < PR; ++r) {
+                            pp[k + 1][r] = dfma(le[r], -u[k + 1], pp[k + 1][r]);
+                            nx[r] = pp[k + 1][r];
+                        }
+                        qpivot<PR>(nx, pos, k + 1, qmin(kl, m - 2 - j), lane, lp, rp, p);
+                    }
+                    // rank-1 update of the panel's later columns
+#pragma unroll
+                    for (int r = 0; r < PR; ++r) {
+#pragma unroll
+                        for (int c = k + 2; c < QB; ++c) pp[c][r] = dfma(le[r], -u[c], pp[c][r]);
+                    }
+                    myp = (lane == k) ? p0 : myp;
                     if (lane == 0) {
-                        s_sig[k] = lp + 64 * rp;
+                        s_sig[k] = lp0 + 64 * rp0;
                         // LDS operations of a wave complete in order: a reader that sees
                         // the step count also sees this step's multipliers
                         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -413,7 +442,7 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
                 const int q = lane + 64 * r;
                 if (q < RP) {
 #pragma unroll
-                    for (int c = 0; c < QB; ++c) pb[q * QS + c] = pp[r][c];
+                    for (int c = 0; c < QB; ++c) pb[q * QS + c] = pp[c][r];
                     const int f = pos[r] < QBIG ? pos[r] : q;
                     s_src[b][f] = q;
                 }
@@ -487,6 +516,28 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
         // ================================================================ update waves
         const int w = wv - 2;
         const int g = lane >> 4, li = lane & 15;
+        // Byte offset of element (r, c0 + x) inside block column c0, x*ld + kv + r - c0 - x
+        // elements, as min(t, kQOob - x*ld*ES) + x*ld*ES with t = (kv + r - c0)*ES - x*ES
+        // taken unsigned: a band row kv + r - c below 0 (c - r > kv) wraps t past 2^31 and
+        // the offset becomes kQOob, as does a row r >= m (row term -2^30); columns past the
+        // matrix lie beyond the resource's columns.  The band's lower edge (r - c <= kl)
+        // holds for every row a block column right of the look-ahead one is updated with.
+        constexpr int ES = (int)sizeof(T);
+        const unsigned ldes = (unsigned)ldi * (unsigned)ES;
+        unsigned xe[4], xl[4], xc[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const unsigned x = (unsigned)drow<T>(g, v);
+            xe[v] = x * ES;
+            xl[v] = x * ldes;
+            xc[v] = (unsigned)kQOob - xl[v];
+        }
+        const unsigned ye = (unsigned)li * ES, yl = (unsigned)li * ldes, yc = (unsigned)kQOob - yl;
+        auto rterm = [&](int r, int c0) -> int { return r < m ? (kv + r - c0) * ES : -(1 << 30); };
+        auto boff = [](int rt_, unsigned e_, unsigned l_, unsigned c_) -> int {
+            const unsigned t = (unsigned)(rt_ - (int)e_);
+            return (int)((t < c_ ? t : c_) + l_);
+        };
         // Apply panel Pp to block column B, reading and writing AB (mode 0) or
         // writing the rows the next panel needs to the look-ahead buffer (tola).
         // Every load of the block column is issued before the first is used
@@ -516,10 +567,10 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int f = 4 * g + i;
-                braw[i] = qbload<T>(rs, f < npp ? offs(j0p + src[f], c0 + li) : kQOob);
+                braw[i] = qbload<T>(rs, f < npp ? boff(rterm(j0p + src[f], c0), ye, yl, yc) : kQOob);
             }
             const T *lv = linv + b * QB * QB + li * QB + 4 * g;
-            v4 u = {T(0), T(0), T(0), T(0)};
+            v4 u = {T(0), T(0), T(0), T(0)}, nu;
             // tiles in chunks of CH (FP64 keeps fewer loads in flight to stay
             // within the register budget); a chunk's stores never overwrite a
             // row a later chunk reads (those come from the row itself or from
@@ -536,13 +587,16 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
                     if (rt < RT) {
                         const int q = src[16 * rt + li];
                         qv[t] = q;
+                        const int rq = rterm(j0p + q, c0);
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) acc[t][v] = qbload<T>(rs, offs(j0p + q, c0 + drow<T>(g, v)));
+                        for (int v = 0; v < 4; ++v) acc[t][v] = qbload<T>(rs, boff(rq, xe[v], xl[v], xc[v]));
                     }
                 }
                 if (ch == 0) {
 #pragma unroll
                     for (int i = 0; i < 4; ++i) u = Mf<T>::mma(lv[i], braw[i], u);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) nu[i] = -u[i];
                 }
 #pragma unroll
                 for (int t = 0; t < CH; ++t) {
@@ -552,15 +606,15 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
                         v4 a = {acc[t][0], acc[t][1], acc[t][2], acc[t][3]};
                         const T *lr = pb + qv[t] * QS;
 #pragma unroll
-                        for (int i = 0; i < 4; ++i) a = Mf<T>::mma(u[i], -lr[drow<T>(g, i)], a);
+                        for (int i = 0; i < 4; ++i) a = Mf<T>::mma(nu[i], lr[drow<T>(g, i)], a);
                         if (f < R) {
                             if (tola) {
 #pragma unroll
                                 for (int v = 0; v < 4; ++v) lab[(f - QB) * QS + drow<T>(g, v)] = a[v];
                             } else {
+                                const int rf = rterm(j0p + f, c0);
 #pragma unroll
-                                for (int v = 0; v < 4; ++v)
-                                    qbstore<T>(rs, offs(j0p + f, c0 + drow<T>(g, v)), a[v]);
+                                for (int v = 0; v < 4; ++v) qbstore<T>(rs, boff(rf, xe[v], xl[v], xc[v]), a[v]);
                             }
                         }
                     }
@@ -571,7 +625,7 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int f = drow<T>(g, v);
-                qbstore<T>(rs, f < npp ? offs(j0p + f, c0 + li) : kQOob, u[v]);
+                qbstore<T>(rs, f < npp ? boff(rterm(j0p + f, c0), ye, yl, yc) : kQOob, u[v]);
             }
             if (tola) {
                 // rows entering: look-ahead buffer positions R-16.. and the next
@@ -610,6 +664,9 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
             v4 u = {T(0), T(0), T(0), T(0)};
 #pragma unroll
             for (int i = 0; i < 4; ++i) u = Mf<T>::mma(lv[i], braw[i], u);
+            v4 nu;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) nu[i] = -u[i];
             T *dstp = pan + ((Pp + 1) & 1) * RP * QS;
             for (int rt = rt0; rt < RT; rt += QUW) {
                 const int f = 16 * rt + li;
@@ -619,7 +676,7 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
                 for (int v = 0; v < 4; ++v) a[v] = lab[q * QS + drow<T>(g, v)];
                 const T *lr = pb + q * QS;
 #pragma unroll
-                for (int i = 0; i < 4; ++i) a = Mf<T>::mma(u[i], -lr[drow<T>(g, i)], a);
+                for (int i = 0; i < 4; ++i) a = Mf<T>::mma(nu[i], lr[drow<T>(g, i)], a);
                 if (f < R) {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) dstp[(f - QB) * QS + drow<T>(g, v)] = a[v];
