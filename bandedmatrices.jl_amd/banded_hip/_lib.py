@@ -82,6 +82,9 @@ SIGNATURES["bh_dsgbsv"] = (C.c_int, [i64, i64, i64, i64, vp, i64, vp, i64, vp, i
 SIGNATURES["bh_gbtrs_blocked_workspace"] = (C.c_int, [C.c_char, i64, i64, i64, C.POINTER(C.c_size_t)])
 for _p in "ds":
     SIGNATURES[f"bh_{_p}gbtrs_blocked"] = (C.c_int, [i64, i64, i64, i64, vp, i64, vp, vp, i64, vp, C.c_size_t])
+SIGNATURES["bh_gbtrs_part_workspace"] = (C.c_int, [C.c_char, i64, i64, i64, i64, C.POINTER(C.c_size_t)])
+for _p in "ds":
+    SIGNATURES[f"bh_{_p}gbtrs_part"] = (C.c_int, [C.c_char, i64, i64, i64, i64, vp, i64, vp, vp, i64, vp, C.c_size_t])
 SIGNATURES["bh_dsgbsv_noalloc"] = (C.c_int, [i64, i64, i64, i64, vp, i64, vp, i64, vp, i64, vp, pi64, pi64])
 
 _lib = None

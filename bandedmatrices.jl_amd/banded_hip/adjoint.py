@@ -46,18 +46,19 @@ def mul_pullback(A: BandedMatrix, x0: DeviceArray, ybar: DeviceArray):
     return Abar, x0bar
 
 
-def loss_and_grad(A: BandedMatrix, b: DeviceArray, w: DeviceArray, x0: DeviceArray):
+def loss_and_grad(A: BandedMatrix, b: DeviceArray, w: DeviceArray, x0: DeviceArray, method: str = "exact"):
     """L = w' A^{-1} b + 1/2 ||A x0||^2 with its gradient: returns (L, dA, db, x).
+    method: the two solves' method (ldiv_'s "exact" or "partitioned").
 
     Forward: band LU, 'N' solve, gbmv.  Backward: 'T' solve with the same
     factors and pivots, then one fused band kernel
     dA_ij = fma(y0_i, x0_j, -(lambda_i x_j))."""
     n = A.shape[0]
     F = lu(A)
-    x = ldiv_(F, b.copy())
+    x = ldiv_(F, b.copy(), method)
     y0 = DeviceArray((n,), A.dtype)
     mul(y0, A, x0)
-    lam = ldiv_(F.transpose(), w.copy())
+    lam = ldiv_(F.transpose(), w.copy(), method)
     dA = BandedMatrix.zeros((n, n), A.bandwidths(), A.dtype)
     _lib.call(f"bh_{_prefix(A.dtype)}gb_band_grad", n, n, A.l, A.u, lam.ptr, x.ptr, y0.ptr, x0.ptr, dA.data.ptr,
               dA.data.ld)

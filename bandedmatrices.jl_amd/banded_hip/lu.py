@@ -120,9 +120,33 @@ def lu(A: BandedMatrix, check: bool = True) -> BandedLU:
     return lu_(pad_for_lu(A), check=check)
 
 
-def ldiv_(F, B: DeviceArray) -> DeviceArray:
+_PART_WS: dict = {}
+
+
+def _part_workspace(prec: str, n: int, kl: int, ku: int, nrhs: int):
+    """Device workspace of the partitioned solve (kept per shape), or None
+    when the bands are outside its range."""
+    key = (prec, n, kl, ku, nrhs)
+    ws = _PART_WS.get(key)
+    if ws is None:
+        nb = C.c_size_t(0)
+        _lib.call("bh_gbtrs_part_workspace", prec.encode(), n, kl, ku, nrhs, C.byref(nb))
+        if nb.value == 0:
+            return None
+        ws = DeviceArray((nb.value,), np.uint8)
+        _PART_WS.clear()
+        _PART_WS[key] = ws
+    return ws
+
+
+def ldiv_(F, B: DeviceArray, method: str = "exact") -> DeviceArray:
     """ldiv!(F, B) in place; F a BandedLU or transpose(F), or a triangular view
-    of a BandedMatrix (UpperTriangular(A) etc., solved with tbsv)."""
+    of a BandedMatrix (UpperTriangular(A) etc., solved with tbsv).
+
+    method="exact": LAPACK's gbtrs operation order on the GPU, bit for bit.
+    method="partitioned": the rows split into blocks solved at once across the
+    GPU (bh_?gbtrs_part, kl <= 31 and kl + ku <= 63): equal to rounding; falls
+    back to the exact solve outside that range."""
     from .symbanded import BandedCholesky, BandedLDLt, cholesky_ldiv_, ldlt_ldiv_
     from .tribanded import _Triangular
 
@@ -146,6 +170,14 @@ def ldiv_(F, B: DeviceArray) -> DeviceArray:
         return B
     nrhs = 1 if len(B.shape) == 1 else B.shape[1]
     p = _prefix(B.dtype)
+    if method == "partitioned":
+        ws = _part_workspace(p, n, F.kl, F.ku, nrhs)
+        if ws is not None:
+            _lib.call(f"bh_{p}gbtrs_part", trans.encode(), n, F.kl, F.ku, nrhs, F.factors.data.ptr,
+                      F.factors.data.ld, F.ipiv_dev.ptr, B.ptr, B.ld, ws.ptr, ws.nbytes)
+            return B
+    elif method != "exact":
+        raise ValueError(f"unknown solve method {method!r}")
     _lib.call(f"bh_{p}gbtrs", trans.encode(), n, F.kl, F.ku, nrhs, F.factors.data.ptr, F.factors.data.ld,
               F.ipiv_dev.ptr, B.ptr, B.ld)
     return B

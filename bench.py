@@ -377,9 +377,19 @@ def bench_adjoint(bh, steps, skip_cpu):
     dA = bh.BandedMatrix.from_host(A, n, l, u)
     db, dw, dx0 = (bh.DeviceArray.from_numpy(v) for v in (b, w, x0))
     bh.loss_and_grad(dA, db, dw, dx0)
+    bh.loss_and_grad(dA, db, dw, dx0, method="partitioned")
     bh.sync()
-    ms = gpu_time(bh, lambda: bh.loss_and_grad(dA, db, dw, dx0), max(2, steps // 2))
-    out = {"n": n, "l": l, "u": u, "ms_loss_and_grad": ms}
+    # the two solves partitioned across the GPU (bh_dgbtrs_part) -- the
+    # headline figure -- and with the bit-exact sequential chain beside it
+    ms = gpu_time(bh, lambda: bh.loss_and_grad(dA, db, dw, dx0, method="partitioned"), max(2, steps // 2))
+    ms_exact = gpu_time(bh, lambda: bh.loss_and_grad(dA, db, dw, dx0), max(2, steps // 2))
+    Lp, gp, lp_, xp = bh.loss_and_grad(dA, db, dw, dx0, method="partitioned")
+    Le, ge, le_, xe = bh.loss_and_grad(dA, db, dw, dx0)
+    gph, geh = gp.data.numpy(), ge.data.numpy()
+    out = {"n": n, "l": l, "u": u, "ms_loss_and_grad": ms, "solves": "partitioned (bh_dgbtrs_part)",
+           "ms_loss_and_grad_exact_solves": ms_exact,
+           "grad_rel_diff_partitioned_vs_exact": float(np.abs(gph - geh).max() / np.abs(geh).max()),
+           "loss_rel_diff_partitioned_vs_exact": abs(Lp - Le) / abs(Le)}
     # the pieces
     F = bh.lu(dA)
     x = db.copy()
@@ -389,9 +399,11 @@ def bench_adjoint(bh, steps, skip_cpu):
     hp = 2 * l + u + 1
     pieces = {
         "lu": (lambda: bh.lu(dA), ((l + u + 1) + hp) * 8.0 * n),
-        "ldiv_N": (lambda: bh.ldiv_(F, x), (hp + 2) * 8.0 * n),
+        "ldiv_N": (lambda: bh.ldiv_(F, x, method="partitioned"), (hp + 2) * 8.0 * n),
+        "ldiv_N_exact": (lambda: bh.ldiv_(F, x), (hp + 2) * 8.0 * n),
         "gbmv": (lambda: bh.mul(y0, dA, dx0), (l + u + 3) * 8.0 * n),
-        "ldiv_T": (lambda: bh.ldiv_(F.transpose(), lam), (hp + 2) * 8.0 * n),
+        "ldiv_T": (lambda: bh.ldiv_(F.transpose(), lam, method="partitioned"), (hp + 2) * 8.0 * n),
+        "ldiv_T_exact": (lambda: bh.ldiv_(F.transpose(), lam), (hp + 2) * 8.0 * n),
         "band_grad": (lambda: _lib.call("bh_dgb_band_grad", n, n, l, u, lam.ptr, x.ptr, y0.ptr, dx0.ptr, G.data.ptr,
                                         G.data.ld), (l + u + 1 + 4) * 8.0 * n),
     }
@@ -403,7 +415,7 @@ def bench_adjoint(bh, steps, skip_cpu):
         per[name] = {"ms": pm, "bytes": nbytes, "GBps": nbytes / (pm * 1e-3) / 1e9,
                      "frac_of_hbm_peak": nbytes / (pm * 1e-3) / 1e9 / HBM_PEAK_GBPS}
     out["pieces"] = per
-    big = max(per, key=lambda k: per[k]["ms"])
+    big = max((k for k in per if not k.endswith("_exact")), key=lambda k: per[k]["ms"])
     out["roofline"] = {"kernel": f"{big} (the longest piece)", "bound": "hbm", "achieved": per[big]["GBps"],
                        "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": per[big]["frac_of_hbm_peak"],
                        "bytes_per_launch": per[big]["bytes"]}
@@ -493,11 +505,17 @@ def bench_plain(bh, steps, skip_cpu):
     f_mv = lambda: bh.mul(y, A, b)
     f_mm = lambda: bh.gbmm(A, A, C)
     f_sv = lambda: bh.solve(A, b)
-    for f in (f_mv, f_mm, f_sv):
+    Fp = bh.lu(A)
+    xb = b.copy()
+    f_lu = lambda: bh.lu(A)
+    f_trs = lambda: bh.ldiv_(Fp, xb, method="partitioned")
+    f_trs_exact = lambda: bh.ldiv_(Fp, xb)
+    for f in (f_mv, f_mm, f_sv, f_lu, f_trs, f_trs_exact):
         f()
     bh.sync()
     out = {"n": n, "l": l, "u": u, "us_gbmv": gpu_time(bh, f_mv, 50) * 1e3, "us_gbmm": gpu_time(bh, f_mm, 50) * 1e3,
-           "us_solve": gpu_time(bh, f_sv, 5) * 1e3}
+           "us_solve": gpu_time(bh, f_sv, 5) * 1e3, "us_lu": gpu_time(bh, f_lu, 5) * 1e3,
+           "us_ldiv_partitioned": gpu_time(bh, f_trs, 20) * 1e3, "us_ldiv_exact": gpu_time(bh, f_trs_exact, 5) * 1e3}
     # FP64 backward error of the GPU solve, and its pivots against OpenBLAS's
     import scipy.linalg.blas as sb
     import scipy.linalg.lapack as la

@@ -156,6 +156,27 @@ int bh_dgbtrs_blocked(bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const double 
 int bh_sgbtrs_blocked(bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const float *AB, bh_int ldab,
                       const bh_int *ipiv, float *B, bh_int ldb, void *work, size_t lwork);
 
+/* Partitioned solve A X = B ('N') or A^T X = B ('T'/'C') with the factors of
+ * bh_?gbtrf: the rows split into blocks of about sqrt(n) solved at once
+ * across the GPU, joined through the linear map each block applies to the
+ * state it takes over (a window of KW > bandwidth values).  Same result as
+ * bh_?gbtrs to rounding, not bit for bit (bh_?gbtrs keeps the bit-exact
+ * sequential chain); for long systems with narrow bands: kl <= 31 and
+ * kl + ku <= 63 (bh_gbtrs_part_workspace gives 0 bytes outside that range,
+ * and the solve returns BH_ERR_WORKSPACE).  work: device workspace of
+ * bh_gbtrs_part_workspace(prec 'd'/'s', n, kl, ku, nrhs) bytes.  The
+ * partitioned result's rounding error grows with the blocks' transfer
+ * matrices; when an entry of one exceeds 1e8 (FP64) / 1e3 (FP32) in
+ * magnitude the call runs bh_?gbtrs instead (B is written once, by whichever
+ * ran).  One host synchronisation per call (that check).
+ * Same job as LAPACK.gbtrs! in ldiv!(::BandedLU, B) and
+ * ldiv!(transpose(::BandedLU), B) (src/banded/linalg.jl:24-47). */
+int bh_gbtrs_part_workspace(char prec, bh_int n, bh_int kl, bh_int ku, bh_int nrhs, size_t *bytes);
+int bh_dgbtrs_part(char trans, bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const double *AB, bh_int ldab,
+                   const bh_int *ipiv, double *B, bh_int ldb, void *work, size_t lwork);
+int bh_sgbtrs_part(char trans, bh_int n, bh_int kl, bh_int ku, bh_int nrhs, const float *AB, bh_int ldab,
+                   const bh_int *ipiv, float *B, bh_int ldb, void *work, size_t lwork);
+
 /* ------------------------------------------------------------- banded QR */
 /* Householder QR in band storage: D is the padded copy (bh_?gbpad with
  * kl = l, ku = u_orig) viewed as bands (l, u) with u = l + u_orig, ldd >= l+u+1,
