@@ -775,6 +775,18 @@ def traffic_probe(bh, name):
         B32 = bh.DeviceArray.from_numpy(np.asfortranarray(B_h.astype(np.float32)))
         _lib.call("bh_sgbtrs_blocked", n, l, u, nrhs, ab.data.ptr, ab.data.ld, ipiv.ptr, B32.ptr, n, work.ptr,
                   nb.value)
+    elif name == "part64":
+        # the adjoint workload's matrix: one factorisation, then three 'N' and
+        # three 'T' partitioned solves
+        n, l, u = 262144, 16, 16
+        rng = np.random.default_rng(1)
+        A = np.asfortranarray(rng.standard_normal((l + u + 1, n)))
+        A[u, :] += 2 * (l + u)
+        F = bh.lu(bh.BandedMatrix.from_host(A, n, l, u))
+        b = bh.DeviceArray.from_numpy(rng.standard_normal(n))
+        for _ in range(3):
+            bh.ldiv_(F, b.copy(), method="partitioned")
+            bh.ldiv_(F.transpose(), b.copy(), method="partitioned")
     bh.sync()
     print(json.dumps({"traffic_probe": name, "copy_bytes": src.nbytes}))
 
