@@ -284,10 +284,11 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
     T *pbuf = reinterpret_cast<T *>(q_raw);  // [2][RP][QS] panel slots after factoring
     T *pan = pbuf + 2 * RP * QS;              // [2][RP][QS] next panel input (position order)
     T *lab = pan + 2 * RP * QS;               // [RP][QS]    look-ahead block column (slot order)
-    T *colb = lab + RP * QS;                  // [QB][64*PR] negated multipliers per step
+    T *colb = lab + RP * QS;                  // [QB][64*PR] multipliers per step
     T *linv = colb + QB * 64 * PR;            // [2][QB][QB] L11^{-1}, row major
     __shared__ int s_src[2][256];             // final position -> slot
     __shared__ int s_sig[QB];                 // pivot slot of each step
+    __shared__ int s_land[2][64];             // stores of the lanes that do not signal
     __shared__ int s_ju[2];
     __shared__ int s_step;
     __shared__ int s_info;
@@ -349,6 +350,9 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
                 for (int c = 0; c < QB; ++c) pp[c][r] = (q < R) ? pin[q * QS + c] : T(0);
             }
             int myp = 0;
+            int slab[PR];  // (slot << 6) | lane, the label the pivot search returns
+#pragma unroll
+            for (int r = 0; r < PR; ++r) slab[r] = (r << 6) | lane;
             // multipliers of a step go to AB column j through a buffer resource over
             // the panel's columns (an inactive slot's store lands out of range)
             const int ncol = qmin(QB, n - j0);
@@ -383,28 +387,34 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
                     // takes position p, the pivot slot takes position k
                     ju = nz ? qmax(ju, qmin(j + ku + (p - k), n - 1)) : ju;
                     info = (!nz && info == 0) ? j + 1 : info;
+                    // a zero pivot relabels nothing: no slot matches the label then,
+                    // and position k maps to itself
+                    const int pn = nz ? p : k;
+                    const int labn = nz ? ((rp << 6) | lp) : -1;
 #pragma unroll
                     for (int r = 0; r < PR; ++r) {
-                        int L = (pos[r] == k) ? p : pos[r];
-                        L = (r == rp && lane == lp) ? k : L;
-                        pos[r] = nz ? L : pos[r];
+                        int L = (pos[r] == k) ? pn : pos[r];
+                        pos[r] = (slab[r] == labn) ? k : L;
                     }
+                    // a zero pivot leaves the column as it is: reciprocal 1, and no
+                    // slot takes a nonzero multiplier
                     const T rcp = nz ? qrcp(pv) : T(1);
+                    const int kmz = nz ? km : 0;
                     T *cb = colb + k * (64 * PR);
                     T le[PR];
 #pragma unroll
                     for (int r = 0; r < PR; ++r) {
                         const int L = pos[r];
-                        const bool act = (L > k) && (L <= k + km);
+                        const bool act = (unsigned)(L - k - 1) < (unsigned)km;  // k < L <= k + km
                         const T raw = pp[k][r];
-                        const T l = nz ? raw * rcp : raw;  // a zero pivot leaves the column as it is
+                        const T l = raw * rcp;
                         const int off = act ? (k * (ldi - 1) + kv + L) * (int)sizeof(T) : kQOob;
                         qbstore<T>(rpan, off, l);
                         pp[k][r] = act ? l : raw;
                         // inactive slots take multiplier 0 (a signed zero can lose its
                         // sign in their rank-1 update)
-                        le[r] = (act && nz) ? l : T(0);
-                        cb[lane + 64 * r] = -le[r];
+                        le[r] = ((unsigned)(L - k - 1) < (unsigned)kmz) ? l : T(0);
+                        cb[lane + 64 * r] = le[r];
                     }
                     const int lp0 = lp, rp0 = rp, p0 = p;
                     if constexpr (k + 1 < QB) {
@@ -425,10 +435,18 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
                         for (int c = k + 2; c < QB; ++c) pp[c][r] = dfma(le[r], -u[c], pp[c][r]);
                     }
                     myp = (lane == k) ? p0 : myp;
-                    if (lane == 0) {
+                    // the step's signal, from lane 0.  FP32: the other lanes store to their
+                    // own landing words, so there is no branch and the step stays one block
+                    // (its reductions interleave with the update); FP64 branches, which
+                    // keeps the step's live registers within the budget.
+                    // LDS operations of a wave complete in order: a reader that sees the
+                    // step count also sees this step's multipliers.
+                    if constexpr (sizeof(T) == 4) {
+                        *(lane == 0 ? &s_sig[k] : &s_land[0][lane]) = lp0 + 64 * rp0;
+                        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                        lds_store_relaxed(lane == 0 ? &s_step : &s_land[1][lane], P * QB + k + 1);
+                    } else if (lane == 0) {
                         s_sig[k] = lp0 + 64 * rp0;
-                        // LDS operations of a wave complete in order: a reader that sees
-                        // the step count also sees this step's multipliers
                         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                         lds_store_relaxed(&s_step, P * QB + k + 1);
                     }
@@ -490,10 +508,10 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
                     }
                     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
                     const int q = s_sig[k];
-                    // row k: X(k, j) = [k == j] - sum_{k' < k} L11(k, k') X(k', j); colb holds -l
+                    // row k: X(k, j) = [k == j] - sum_{k' < k} L11(k, k') X(k', j); colb holds l
                     T x = (lane == k) ? T(1) : T(0);
 #pragma unroll
-                    for (int kk = 0; kk < k; ++kk) x = dfma(colb[kk * (64 * PR) + q], X[kk], x);
+                    for (int kk = 0; kk < k; ++kk) x = dfma(-colb[kk * (64 * PR) + q], X[kk], x);
                     X[k] = x;
                 }
             });
@@ -746,11 +764,12 @@ size_t mfma_lds_bytes(i64 kl, i64 elem) {
 }
 
 // The blocked MFMA band LU, or -1 when the band does not fit it (kl + 16 rows
-// in at most four slots per lane and the LDS buffers within 160 KiB).
+// in at most four slots per lane and the LDS buffers, with the kernel's
+// static words (under 4 KiB), within 160 KiB).
 template <typename T>
 int mfma_gbtrf_launch(i64 m, i64 n, i64 kl, i64 ku, T *ab, i64 ldab, i64 *ipiv, i64 *dinfo, hipStream_t st) {
     const size_t lds = mfma_lds_bytes(kl, sizeof(T));
-    if (lds == 0 || lds > 160 * 1024 || m >= (1ll << 29) || n >= (1ll << 29) || kl + ku >= (1 << 20) ||
+    if (lds == 0 || lds + 4096 > 160 * 1024 || m >= (1ll << 29) || n >= (1ll << 29) || kl + ku >= (1 << 20) ||
         (i64)QB * ldab * (i64)sizeof(T) >= (1ll << 31) || ldab * (i64)sizeof(T) >= (1ll << 24))
         return -1;
     const int R = (int)kl + QB, RT = (R + 15) / 16, RP = 16 * RT;
