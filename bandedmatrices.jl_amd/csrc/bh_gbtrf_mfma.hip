@@ -24,14 +24,17 @@
 //   * the HELPER WAVE follows the panel wave one step behind and builds
 //     L11^{-1} (16x16 unit lower) row by row from the pivot rows'
 //     multipliers; it also writes the panel's U11 block to AB.
-//   * eight UPDATE WAVES apply the panel to the trailing block columns (16
+//   * ten UPDATE WAVES apply the panel to the trailing block columns (16
 //     columns each) as xGBTRF does, on MFMA 16x16x4:
 //         U12 = L11^{-1} (P A)[0:16]            (4 MFMA)
 //         A22 = (P A)[16:R] - L21 U12           (4 MFMA per 16-row tile)
 //     reading the rows of P A straight from AB through the slot map (a
 //     gather: the permutation costs nothing) and writing U12 and A22 back
-//     in final order.  Block column B is always handled by update wave
-//     B mod 8, so its loads see its own earlier stores.
+//     in final order.  A workgroup's waves go to the four SIMDs in turn, so
+//     waves 4 and 8 share the panel wave's SIMD: they take look-ahead tiles
+//     only, and the other eight update waves the block columns, block
+//     column B always the one of rank B mod 8, so its loads see its own
+//     earlier stores.
 // Look-ahead: block column P+1 (the next panel) goes through LDS only.  In
 // the phase where the panel wave factors panel P, the update waves apply
 // panel P-1 to block columns P+1.. (block column P+1 into the LDS
@@ -52,7 +55,8 @@ namespace bh {
 namespace {
 
 constexpr int QB = 16;       // panel width = block-column width
-constexpr int QUW = 8;       // update waves
+constexpr int QUW = 10;      // update waves
+constexpr int QP1 = 8;       // of which take block columns (the rest only look-ahead tiles)
 constexpr int QWAVES = QUW + 2;
 constexpr int QBIG = 1 << 29;  // position of a slot that holds no row
 
@@ -435,13 +439,14 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
                         for (int c = k + 2; c < QB; ++c) pp[c][r] = dfma(le[r], -u[c], pp[c][r]);
                     }
                     myp = (lane == k) ? p0 : myp;
-                    // the step's signal, from lane 0.  FP32: the other lanes store to their
+                    // the step's signal, from lane 0.  FP32 and FP64 with up to two slots: the
+                    // other lanes store to their
                     // own landing words, so there is no branch and the step stays one block
-                    // (its reductions interleave with the update); FP64 branches, which
-                    // keeps the step's live registers within the budget.
+                    // (its reductions interleave with the update); FP64 with three or four
+                    // slots branches, which keeps the step's live registers within budget.
                     // LDS operations of a wave complete in order: a reader that sees the
                     // step count also sees this step's multipliers.
-                    if constexpr (sizeof(T) == 4) {
+                    if constexpr (sizeof(T) == 4 || PR <= 2) {
                         *(lane == 0 ? &s_sig[k] : &s_land[0][lane]) = lp0 + 64 * rp0;
                         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                         lds_store_relaxed(lane == 0 ? &s_step : &s_land[1][lane], P * QB + k + 1);
@@ -533,6 +538,8 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
     } else {
         // ================================================================ update waves
         const int w = wv - 2;
+        // rank among the waves that take block columns (-1: look-ahead tiles only)
+        const int w1 = (wv & 3) == 0 ? -1 : w - (w > 2 ? 1 : 0) - (w > 6 ? 1 : 0);
         const int g = lane >> 4, li = lane & 15;
         // Byte offset of element (r, c0 + x) inside block column c0, x*ld + kv + r - c0 - x
         // elements, as min(t, kQOob - x*ld*ES) + x*ld*ES with t = (kv + r - c0)*ES - x*ES
@@ -717,7 +724,7 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
                 const bool la = (P < NP) && ((P + 1) * QB < n);
                 const int Bend = la ? qmax(Bmax, P + 1) : Bmax;
                 for (int B = P + 1; B <= Bend; ++B) {
-                    if (B % QUW != w) continue;
+                    if (B % QP1 != w1) continue;
                     update_glb(Pp, B, la && B == P + 1);
                 }
             }
