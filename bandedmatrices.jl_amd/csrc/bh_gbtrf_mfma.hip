@@ -730,8 +730,9 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
             }
             if (P == NP) break;
             lds_barrier();  // B1
-            // ---- phase 2: panel P onto the look-ahead block column P+1
-            if ((P + 1) * QB < n) update_la(P, 1 + w, w == 0);
+            // ---- phase 2: panel P onto the look-ahead block column P+1 (waves
+            // without a tile stay out of the LDS traffic)
+            if ((P + 1) * QB < n && (1 + w < RT || w == 0)) update_la(P, 1 + w, w == 0);
             lds_barrier();  // B2
         }
     }
