@@ -525,8 +525,10 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
 #pragma unroll
                 for (int i = 0; i < QB; ++i) li[i * QB + lane] = X[i];
             }
-            if (P >= 1) write_u11(P - 1);
             lds_barrier();  // B1
+            // U11 of the previous panel while the update waves work on the
+            // look-ahead block (its slots and slot map are replaced only after B2)
+            if (P >= 1) write_u11(P - 1);
             lds_barrier();  // B2
         }
         write_u11(NP - 1);
