@@ -346,12 +346,21 @@ __global__ void __launch_bounds__(64 * QWAVES) gbtrf_mfma_kernel(QArgs<T> s) {
             // per column, so the pivot slot's value is picked by a uniform index)
             typename QCol<T, PR>::type pp[QB];
             int pos[PR];
+            // 16-byte LDS reads (a row's 16 entries are contiguous, rows 16-byte aligned);
+            // slots past the window read a clamped row and take zeros
+            typedef T QV16 __attribute__((ext_vector_type(16 / sizeof(T))));
+            constexpr int VW = 16 / (int)sizeof(T);
 #pragma unroll
             for (int r = 0; r < PR; ++r) {
                 const int q = lane + 64 * r;
                 pos[r] = (q < R && j0 + q < m) ? q : QBIG;
+                const T *row = pin + qmin(q, RP - 1) * QS;
 #pragma unroll
-                for (int c = 0; c < QB; ++c) pp[c][r] = (q < R) ? pin[q * QS + c] : T(0);
+                for (int c = 0; c < QB; c += VW) {
+                    const QV16 v = *reinterpret_cast<const QV16 *>(row + c);
+#pragma unroll
+                    for (int e = 0; e < VW; ++e) pp[c + e][r] = (q < R) ? v[e] : T(0);
+                }
             }
             int myp = 0;
             int slab[PR];  // (slot << 6) | lane, the label the pivot search returns
