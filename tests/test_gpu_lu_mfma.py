@@ -1,6 +1,9 @@
 """GPU parity: the MFMA-blocked band LU -- for ku >= 65, kl >= 32 (the
 regime in which the reference's LAPACK.gbtrf! runs blocked xGBTRF) and for
-long systems (min(m, n) >= 16384) with kl, ku >= 8.
+long systems (min(m, n) >= 16384) with kl, ku >= 8.  A planted first column
+checks the pivot search's tie rules: magnitudes equal in their high 32 bits
+(the FP64 search's second wave maximum decides) and exactly equal ones (the smallest
+position wins).
 
 Pivots (ipiv) and info must equal the oracle's (xGBTF2 restated, which picks
 the same pivots as xGBTRF); factors must agree to rounding (FP64: 1e-12 of
@@ -80,3 +83,30 @@ def test_gbtrf_mfma_solve(bh, oracle, dtype, n, kl, ku):
     r = b.astype(np.float64) - Ad @ x.astype(np.float64)
     berr = np.abs(r).max() / (np.abs(Ad).sum(1).max() * np.abs(x).max() + np.abs(b).max())
     assert berr <= (1e-14 if dtype == np.float64 else 1e-5), berr
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("bump,expect", [(2.0 ** -40, 5), (0.0, 2)])
+def test_gbtrf_mfma_pivot_ties(bh, oracle, dtype, bump, expect):
+    """Column 0 holds 7 at rows 2 and 5, the second one larger by a relative
+    2^-40 (equal high words in FP64) or equal; every other entry is below 1."""
+    n, kl, ku = 16384, 8, 8
+    rng = np.random.default_rng(11)
+    A = band(rng, kl, ku, n, np.float64, dist="u") - 0.5
+    A[ku + 2, 0] = 7.0
+    A[ku + 5, 0] = 7.0 * (1.0 + bump)
+    A = A.astype(dtype)
+    ab = oracle.pad_for_lu(A, n, n, kl, ku)
+    fref, pref, iref = oracle.gbtf2(n, n, kl, ku, ab.copy())
+    dab = bh.DeviceArray.from_numpy(ab)
+    ipiv = bh.DeviceArray((n,), np.int64)
+    info = C.c_int64(-7)
+    p = "d" if dtype == np.float64 else "s"
+    bh._lib.call(f"bh_{p}gbtrf", n, n, kl, ku, dab.ptr, dab.ld, ipiv.ptr, C.byref(info))
+    piv = ipiv.numpy()
+    exp_row = expect if (dtype == np.float64 or bump == 0.0) else 2  # in FP32 the bump rounds away
+    assert pref[0] == exp_row + 1
+    assert piv[0] == pref[0]
+    assert info.value == iref
+    same = int(np.sum(piv == pref))
+    assert same >= (n if dtype == np.float64 else n - 2), f"pivots {same}/{n}"
