@@ -46,9 +46,10 @@ def mul_pullback(A: BandedMatrix, x0: DeviceArray, ybar: DeviceArray):
     return Abar, x0bar
 
 
-def loss_and_grad(A: BandedMatrix, b: DeviceArray, w: DeviceArray, x0: DeviceArray, method: str = "exact"):
+def loss_and_grad(A: BandedMatrix, b: DeviceArray, w: DeviceArray, x0: DeviceArray, method: str = "auto"):
     """L = w' A^{-1} b + 1/2 ||A x0||^2 with its gradient: returns (L, dA, db, x).
-    method: the two solves' method (ldiv_'s "exact" or "partitioned").
+    method: the two solves' method (ldiv_'s "auto" -- partitioned from n = 4096
+    on --, "exact" or "partitioned").
 
     Forward: band LU, 'N' solve, gbmv.  Backward: 'T' solve with the same
     factors and pivots, then one fused band kernel

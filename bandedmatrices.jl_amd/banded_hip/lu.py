@@ -146,7 +146,8 @@ def ldiv_(F, B: DeviceArray, method: str = "exact") -> DeviceArray:
     method="exact": LAPACK's gbtrs operation order on the GPU, bit for bit.
     method="partitioned": the rows split into blocks solved at once across the
     GPU (bh_?gbtrs_part, kl <= 31 and kl + ku <= 63): equal to rounding; falls
-    back to the exact solve outside that range."""
+    back to the exact solve outside that range.
+    method="auto": partitioned from n = 4096 rows on, exact below."""
     from .symbanded import BandedCholesky, BandedLDLt, cholesky_ldiv_, ldlt_ldiv_
     from .tribanded import _Triangular
 
@@ -170,6 +171,8 @@ def ldiv_(F, B: DeviceArray, method: str = "exact") -> DeviceArray:
         return B
     nrhs = 1 if len(B.shape) == 1 else B.shape[1]
     p = _prefix(B.dtype)
+    if method == "auto":
+        method = "partitioned" if n >= 4096 else "exact"
     if method == "partitioned":
         ws = _part_workspace(p, n, F.kl, F.ku, nrhs)
         if ws is not None:
@@ -183,12 +186,18 @@ def ldiv_(F, B: DeviceArray, method: str = "exact") -> DeviceArray:
     return B
 
 
-def solve(A: BandedMatrix, b: DeviceArray) -> DeviceArray:
-    """A \\ b: checksquare, factorize, solve a copy of b (src/banded/linalg.jl:5-9)."""
+def solve(A: BandedMatrix, b: DeviceArray, method: str = "auto") -> DeviceArray:
+    """A \\ b: checksquare, factorize, solve a copy of b (src/banded/linalg.jl:5-9).
+
+    The factorisation is LAPACK's (same pivots); the solve takes ldiv_'s
+    `method`: by default the partitioned solve for long systems with narrow
+    bands (n >= 4096, backward error at rounding level, not bit for bit) and
+    the bit-exact one otherwise; method="exact" reproduces the reference's
+    LAPACK solve bit for bit."""
     m, n = A.shape
     if m != n:
         raise DimensionMismatch(f"matrix is not square: dimensions are {A.shape}")
     if b.shape[0] != m:
         raise DimensionMismatch(f"b has length {b.shape[0]}, A has {m} rows")
     x = b.copy()
-    return ldiv_(lu(A), x)
+    return ldiv_(lu(A), x, method)

@@ -139,6 +139,10 @@ def test_solve_backward_error_and_no_alias(bh, oracle):
         assert berr <= 1e-12
         if M is not None:
             np.testing.assert_allclose(x, np.linalg.solve(M, b), rtol=1e-8, atol=1e-10)
+        # from n = 4096 on A\b solves partitioned; method="exact" keeps LAPACK's
+        # solve, and both agree to rounding
+        xe = bh.solve(bh.BandedMatrix.from_host(A, n, kl, ku), db, method="exact").numpy()
+        assert np.abs(x - xe).max() <= 1e-10 * np.abs(xe).max()
 
 
 def test_solve_errors(bh):
