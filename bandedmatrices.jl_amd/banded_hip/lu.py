@@ -121,6 +121,23 @@ def lu(A: BandedMatrix, check: bool = True) -> BandedLU:
 
 
 _PART_WS: dict = {}
+_BLK_WS: dict = {}
+
+
+def _blocked_workspace(prec: str, n: int, kl: int, ku: int):
+    """Device workspace of the MFMA-blocked solve (kept per shape), or None
+    when the bands are outside its range."""
+    key = (prec, n, kl, ku)
+    ws = _BLK_WS.get(key)
+    if ws is None:
+        nb = C.c_size_t(0)
+        _lib.call("bh_gbtrs_blocked_workspace", prec.encode(), n, kl, ku, C.byref(nb))
+        if nb.value == 0:
+            return None
+        ws = DeviceArray((nb.value,), np.uint8)
+        _BLK_WS.clear()
+        _BLK_WS[key] = ws
+    return ws
 
 
 def _part_workspace(prec: str, n: int, kl: int, ku: int, nrhs: int):
@@ -147,7 +164,11 @@ def ldiv_(F, B: DeviceArray, method: str = "exact") -> DeviceArray:
     method="partitioned": the rows split into blocks solved at once across the
     GPU (bh_?gbtrs_part, kl <= 31 and kl + ku <= 63): equal to rounding; falls
     back to the exact solve outside that range.
-    method="auto": partitioned from n = 4096 rows on, exact below."""
+    method="blocked": 'N' only, panels of 32 rows on the matrix cores
+    (bh_?gbtrs_blocked): equal to rounding; for wide bands; falls back to the
+    exact solve for 'T' or outside its range.
+    method="auto": from n = 4096 rows on, partitioned for narrow bands (kl <= 31)
+    and blocked for wider ones ('N'); exact below."""
     from .symbanded import BandedCholesky, BandedLDLt, cholesky_ldiv_, ldlt_ldiv_
     from .tribanded import _Triangular
 
@@ -172,7 +193,14 @@ def ldiv_(F, B: DeviceArray, method: str = "exact") -> DeviceArray:
     nrhs = 1 if len(B.shape) == 1 else B.shape[1]
     p = _prefix(B.dtype)
     if method == "auto":
-        method = "partitioned" if n >= 4096 else "exact"
+        method = "exact" if n < 4096 else ("partitioned" if F.kl <= 31 else "blocked")
+    if method == "blocked":
+        ws = _blocked_workspace(p, n, F.kl, F.ku) if trans == "N" else None
+        if ws is not None:
+            _lib.call(f"bh_{p}gbtrs_blocked", n, F.kl, F.ku, nrhs, F.factors.data.ptr, F.factors.data.ld,
+                      F.ipiv_dev.ptr, B.ptr, B.ld, ws.ptr, ws.nbytes)
+            return B
+        method = "exact"
     if method == "partitioned":
         ws = _part_workspace(p, n, F.kl, F.ku, nrhs)
         if ws is not None:
@@ -190,10 +218,10 @@ def solve(A: BandedMatrix, b: DeviceArray, method: str = "auto") -> DeviceArray:
     """A \\ b: checksquare, factorize, solve a copy of b (src/banded/linalg.jl:5-9).
 
     The factorisation is LAPACK's (same pivots); the solve takes ldiv_'s
-    `method`: by default the partitioned solve for long systems with narrow
-    bands (n >= 4096, backward error at rounding level, not bit for bit) and
-    the bit-exact one otherwise; method="exact" reproduces the reference's
-    LAPACK solve bit for bit."""
+    `method`: by default ("auto") the partitioned solve for long systems with
+    narrow bands and the MFMA-blocked one for wide bands (n >= 4096; backward
+    error at rounding level, not bit for bit), the bit-exact one otherwise;
+    method="exact" reproduces the reference's LAPACK solve bit for bit."""
     m, n = A.shape
     if m != n:
         raise DimensionMismatch(f"matrix is not square: dimensions are {A.shape}")

@@ -95,3 +95,22 @@ def test_blocked_solve_edges(bh):
     # a short workspace is refused before any device work
     _lib.call("bh_gbtrs_blocked_workspace", b"d", 100, 3, 3, C.byref(nb))
     assert L.bh_dgbtrs_blocked(100, 3, 3, 1, None, 10, None, None, 100, None, nb.value - 1) == -1004
+
+
+def test_blocked_solve_through_ldiv(bh):
+    """ldiv_(F, B, method="blocked") routes 'N' to the blocked solve (equal to
+    rounding), 'T' to the exact one; "auto" takes the blocked solve for wide
+    bands from n = 4096 on."""
+    rng = np.random.default_rng(9)
+    n, kl, ku = 5000, 64, 64
+    A = band(rng, kl, ku, n, dist="n")
+    A[ku, :] += 2.0 * (kl + ku)
+    F = bh.lu(bh.BandedMatrix.from_host(A, n, kl, ku))
+    B = np.asfortranarray(rng.standard_normal((n, 3)))
+    xe = bh.ldiv_(F, bh.DeviceArray.from_numpy(B)).numpy()
+    for method in ("blocked", "auto"):
+        xb = bh.ldiv_(F, bh.DeviceArray.from_numpy(B), method=method).numpy()
+        assert np.abs(xb - xe).max() <= 1e-11 * np.abs(xe).max()
+    te = bh.ldiv_(F.transpose(), bh.DeviceArray.from_numpy(B)).numpy()
+    tb = bh.ldiv_(F.transpose(), bh.DeviceArray.from_numpy(B), method="blocked").numpy()
+    assert np.array_equal(te, tb)
