@@ -28,6 +28,10 @@ FLAGS = [
 ]
 
 
+_HEAVY = ("gbtrs_ring_", "gbtrs_part_up", "gbtrs_part_lo", "lu_lane_", "gbmv_slab", "tbsv_", "lu_small",
+          "gbtrf_mfma", "gbqr", "gbtrf_reg")
+
+
 def _compile(src: str) -> str:
     obj = os.path.join(OBJ, os.path.basename(src).replace(".hip", ".o"))
     cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
@@ -53,6 +57,9 @@ def build(jobs: int = 8, verbose: bool = False, incremental: bool = False) -> st
     hdrs = glob.glob(os.path.join(CSRC, "*.hpp")) + glob.glob(os.path.join(HERE, "..", "include", "*.h"))
     hm = max((os.path.getmtime(h) for h in hdrs), default=0.0)
     todo = [s for s in srcs if _stale(s, hm)] if incremental else srcs
+    # the longest compiles start first (the units that instantiate many kernel
+    # variants), so the parallel build ends with the short ones
+    todo = sorted(todo, key=lambda s: not any(k in os.path.basename(s) for k in _HEAVY))
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_compile, todo))
     objs = [os.path.join(OBJ, os.path.basename(s).replace(".hip", ".o")) for s in srcs]

@@ -29,7 +29,7 @@
 
 namespace bh {
 
-namespace {
+namespace part {
 
 enum { kLN = 0, kUN = 1, kUT = 2, kLT = 3 };
 constexpr int kPartNR = 16;  // right-hand sides per wave
@@ -360,7 +360,7 @@ __global__ void __launch_bounds__(256) part_chain(PartArgs<T> a, int RB) {
     if (tid < KW * RB) atomicMax(a.growth, __builtin_bit_cast(unsigned, g));
 }
 
-__global__ void part_clear(unsigned *g) {
+static __global__ void part_clear(unsigned *g) {
     if (threadIdx.x < 2) g[threadIdx.x] = 0u;
 }
 
@@ -375,7 +375,7 @@ struct PartPlan {
     int KW, M, p;
 };
 
-PartPlan part_plan(i64 n, int need, int maxkw) {
+inline PartPlan part_plan(i64 n, int need, int maxkw) {
     PartPlan pl{0, 0, 0};
     for (int kw : kPartKW)
         if (kw >= need && kw <= maxkw) {
@@ -429,6 +429,15 @@ int part_sweep_kw(PartArgs<T> a, bool head, hipStream_t st) {
     return BH_OK;
 }
 
+// The 64-wide U sweeps are instantiated in units of their own
+// (bh_gbtrs_part_up64_{d,s}.hip) to spread the compile.
+#ifndef BH_PART_KW64
+extern template int part_sweep_kw<double, 64, kUN>(PartArgs<double>, bool, hipStream_t);
+extern template int part_sweep_kw<double, 64, kUT>(PartArgs<double>, bool, hipStream_t);
+extern template int part_sweep_kw<float, 64, kUN>(PartArgs<float>, bool, hipStream_t);
+extern template int part_sweep_kw<float, 64, kUT>(PartArgs<float>, bool, hipStream_t);
+#endif
+
 // launches 1 and 2 (head) or 3 of one sweep
 template <typename T, int OP>
 int part_sweep(const PartPlan &pl, PartArgs<T> a, bool head, hipStream_t st) {
@@ -448,18 +457,35 @@ int part_sweep(const PartPlan &pl, PartArgs<T> a, bool head, hipStream_t st) {
     }
     if constexpr (!POp<OP>::lower) {
         switch (pl.KW) {
+            // the L sweeps take windows up to 32 wide (kl <= 31)
             case 40:
-                return part_sweep_kw<T, 40, OP>(a, head, st);
+                if constexpr (!POp<OP>::lower) return part_sweep_kw<T, 40, OP>(a, head, st);
+                break;
             case 48:
-                return part_sweep_kw<T, 48, OP>(a, head, st);
+                if constexpr (!POp<OP>::lower) return part_sweep_kw<T, 48, OP>(a, head, st);
+                break;
             case 64:
-                return part_sweep_kw<T, 64, OP>(a, head, st);
+                if constexpr (!POp<OP>::lower) return part_sweep_kw<T, 64, OP>(a, head, st);
+                break;
             default:
                 break;
         }
     }
     return BH_ERR_INTERNAL;
 }
+
+// The sweeps are instantiated in their own units (bh_gbtrs_part_{lo,up}_{d,s}.hip,
+// compiled in parallel); the units that only call them see these declarations.
+#ifndef BH_PART_INSTANTIATE
+extern template int part_sweep<double, kLN>(const PartPlan &, PartArgs<double>, bool, hipStream_t);
+extern template int part_sweep<double, kLT>(const PartPlan &, PartArgs<double>, bool, hipStream_t);
+extern template int part_sweep<double, kUN>(const PartPlan &, PartArgs<double>, bool, hipStream_t);
+extern template int part_sweep<double, kUT>(const PartPlan &, PartArgs<double>, bool, hipStream_t);
+extern template int part_sweep<float, kLN>(const PartPlan &, PartArgs<float>, bool, hipStream_t);
+extern template int part_sweep<float, kLT>(const PartPlan &, PartArgs<float>, bool, hipStream_t);
+extern template int part_sweep<float, kUN>(const PartPlan &, PartArgs<float>, bool, hipStream_t);
+extern template int part_sweep<float, kUT>(const PartPlan &, PartArgs<float>, bool, hipStream_t);
+#endif
 
 // Largest |entry| of the blocks' transfer matrices A_i above which the
 // partitioned result is not trusted (its rounding error grows with them) and
@@ -533,11 +559,12 @@ int part_solve(char trans, i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 lda
     return tr ? part_sweep<T, kLT>(p2, a, false, st) : part_sweep<T, kUN>(p2, a, false, st);
 }
 
-}  // namespace
+}  // namespace part
 
 }  // namespace bh
 
 using namespace bh;
+using namespace bh::part;
 
 template <typename T>
 int gbtrs_part_impl(char trans, i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 ldab, const i64 *ipiv,

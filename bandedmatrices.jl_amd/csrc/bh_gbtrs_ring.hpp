@@ -1,5 +1,8 @@
-// bh_gbtrs_ring.hip -- band LU solve 'N' with one wave per right-hand side,
-// the vector held in a lane ring (no per-step lane shifts).
+// bh_gbtrs_ring.hpp -- band LU solve 'N' and 'T' with one wave per right-hand
+// side, the vector held in a lane ring (no per-step lane shifts).  The
+// launchers are instantiated one per unit (bh_gbtrs_ring_{n,t}_{d,s}.hip) so
+// the fully unrolled kernels compile in parallel.
+#pragma once
 //
 // Restates LAPACK xGBTRS 'N' (ldiv!(::BandedLU, B), src/banded/linalg.jl:24-30)
 // with its rounding, so results are bit-identical:
@@ -402,14 +405,5 @@ int ring_gbtrs_t_launch(i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 ld, co
 #undef BH_RING_T_CASE
     return -1;
 }
-
-template int ring_gbtrs_launch<double>(i64, i64, i64, i64, const double *, i64, const i64 *, double *, i64,
-                                       hipStream_t);
-template int ring_gbtrs_launch<float>(i64, i64, i64, i64, const float *, i64, const i64 *, float *, i64,
-                                      hipStream_t);
-template int ring_gbtrs_t_launch<double>(i64, i64, i64, i64, const double *, i64, const i64 *, double *, i64,
-                                         hipStream_t);
-template int ring_gbtrs_t_launch<float>(i64, i64, i64, i64, const float *, i64, const i64 *, float *, i64,
-                                        hipStream_t);
 
 }  // namespace bh
