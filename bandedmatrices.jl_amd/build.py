@@ -28,7 +28,7 @@ FLAGS = [
 ]
 
 
-_HEAVY = ("gbtrs_ring_", "gbtrs_part_up", "gbtrs_part_lo", "lu_lane_", "gbmv_slab", "tbsv_", "lu_small",
+_HEAVY = ("gbtrs_ring_", "gbtrs_part_up", "gbtrs_part_lo", "lu_lane_", "gbmv_slab_", "tbsv_", "lu_small",
           "gbtrf_mfma", "gbqr", "gbtrf_reg")
 
 

@@ -20,6 +20,10 @@
 namespace bh {
 
 constexpr int kMmThreads = 256;
+// entries per thread computed before their stores issue (measured at n = 2^20,
+// (8,8) x (4,4): 105 us FP64 at 4, 115 us at 8, 108 us at 2, 135 us storing
+// each entry at once)
+constexpr int kMmBatch = 4;
 
 template <typename T>
 struct GbmmArgs {
@@ -35,9 +39,17 @@ struct GbmmArgs {
 };
 
 __host__ __device__ inline i64 odd_stride(i64 h) { return (h % 2 == 0) ? h + 1 : h; }
+template <typename I>
+__device__ __forceinline__ I mm_min(I a, I b) {
+    return a < b ? a : b;
+}
+template <typename I>
+__device__ __forceinline__ I mm_max(I a, I b) {
+    return a > b ? a : b;
+}
 
 // One workgroup per NC consecutive columns of C.
-template <typename T>
+template <typename T, typename I, int NE>
 __global__ void __launch_bounds__(kMmThreads) gbmm_kernel(GbmmArgs<T> p, i64 NC) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const i64 hA = p.Al + p.Au + 1, hB = p.Bl + p.Bu + 1, hC = p.Ctl + p.Ctu + 1;
@@ -51,69 +63,118 @@ __global__ void __launch_bounds__(kMmThreads) gbmm_kernel(GbmmArgs<T> p, i64 NC)
     // Staging: when the band storage has no padding (ld == h) and the LDS
     // stride equals h, the columns are one contiguous run on both sides and
     // are copied flat; otherwise element (column, row) indices come from a
-    // 32-bit division (the slab sizes are small).
-    const int nB = (int)((j1 - j0) * hB), nA = (int)((v1 - v0) * hA);
+    // 32-bit division (the slab sizes are small).  Each thread issues a whole
+    // batch of loads (UB of B, UA of A; indices past the slab are clamped to
+    // its last element, so every load is in bounds and none sits in a branch)
+    // before the batch's LDS stores, so the workgroup waits for global memory
+    // once per batch rather than once per element.
+    const int nB = (int)((j1 - j0) * hB), nA = (int)imax(0, (v1 - v0) * hA);
     const int hBi = (int)hB, hAi = (int)hA, SBi = (int)SB, SAi = (int)SA;
-    if (p.ldb == hB && SB == hB) {
-        const T *src = p.B + j0 * p.ldb;
-        for (int e = threadIdx.x; e < nB; e += blockDim.x) sB[e] = p.alpha * src[e];  // alpha*b, as the reference's gbmv
-    } else {
-        for (int e = threadIdx.x; e < nB; e += blockDim.x) {
-            const int c = e / hBi, t = e - c * hBi;
-            sB[c * SBi + t] = p.alpha * p.B[(j0 + c) * p.ldb + t];
+    const bool flatB = p.ldb == hB && SB == hB, flatA = p.lda == hA && SA == hA;
+    const T *srcB = p.B + j0 * p.ldb, *srcA = p.A + v0 * p.lda;
+    auto offB = [&](int e) { return flatB ? (i64)e : (i64)(e / hBi) * p.ldb + (e % hBi); };
+    auto offA = [&](int e) { return flatA ? (i64)e : (i64)(e / hAi) * p.lda + (e % hAi); };
+    auto dstB = [&](int e) { return flatB ? e : (e / hBi) * SBi + (e % hBi); };
+    auto dstA = [&](int e) { return flatA ? e : (e / hAi) * SAi + (e % hAi); };
+    constexpr int UB = 4, UA = 8;
+    const int bt = (int)blockDim.x, tid = (int)threadIdx.x;
+    const int iters = (int)imax(cdiv(nB, UB * bt), cdiv(nA, UA * bt));
+    for (int it = 0; it < iters; ++it) {
+        T vb[UB], va[UA];
+#pragma unroll
+        for (int q = 0; q < UB; ++q) vb[q] = srcB[offB((int)imin((it * UB + q) * bt + tid, nB - 1))];
+        if (nA > 0) {
+#pragma unroll
+            for (int q = 0; q < UA; ++q) va[q] = srcA[offA((int)imin((it * UA + q) * bt + tid, nA - 1))];
         }
-    }
-    if (p.lda == hA && SA == hA) {
-        const T *src = p.A + v0 * p.lda;
-        for (int e = threadIdx.x; e < nA; e += blockDim.x) sA[e] = src[e];
-    } else {
-        for (int e = threadIdx.x; e < nA; e += blockDim.x) {
-            const int c = e / hAi, t = e - c * hAi;
-            sA[c * SAi + t] = p.A[(v0 + c) * p.lda + t];
+#pragma unroll
+        for (int q = 0; q < UB; ++q) {
+            const int e = (it * UB + q) * bt + tid;
+            if (e < nB) sB[dstB(e)] = p.alpha * vb[q];  // alpha*b, as the reference's gbmv
+        }
+#pragma unroll
+        for (int q = 0; q < UA; ++q) {
+            const int e = (it * UA + q) * bt + tid;
+            if (e < nA) sA[dstA(e)] = va[q];
         }
     }
     __syncthreads();
 
-    const i64 shift = p.Ctu - p.Cu;  // first stored row of the product bands
+    // The entries: thread tid takes e = tid, tid + bt, ... of the NC x hC
+    // block (e = column * hC + row, so the stores are contiguous), stepping
+    // (column, row) without a division; sizes in I (int when every dimension
+    // is below 2^30, which keeps the index arithmetic 32-bit).  A batch of NE
+    // entries is computed into registers before the batch's stores issue:
+    // a store counts against the same wait counter as a load, and storing
+    // each entry before computing the next made every entry wait for the
+    // previous entry's store to complete.
+    const I shift = (I)(p.Ctu - p.Cu);  // first stored row of the product bands
+    const I Ctu = (I)p.Ctu, Cl = (I)p.Cl, Cu = (I)p.Cu, Al = (I)p.Al, Au = (I)p.Au, Bl = (I)p.Bl, Bu = (I)p.Bu;
+    const I n = (I)p.n, k = (I)p.k, m = (I)p.m, jb = (I)j0, vb0 = (I)v0;
+    const I rowlast = Ctu + Cl, jprod = k + Bu, jscale = mm_min(m, n + Cu);
+    const bool b0 = p.beta == T(0);
     const int hCi = (int)hC, nC = (int)((j1 - j0) * hC);
-    for (int e = threadIdx.x; e < nC; e += blockDim.x) {
-        const int jj = e / hCi;
-        const i64 r = e - jj * hCi, j = j0 + jj;
-        T *cp = p.C + j * p.ldc + r;
-        const bool extra = (r < shift) || (r > p.Ctu + p.Cl);
-        if (extra) {
-            *cp = (p.beta == T(0)) ? T(0) : p.beta * *cp;
-            continue;
+    const int sj = bt / hCi, sr = bt - sj * hCi, sa1 = SAi - 1;
+    int jj = tid / hCi, r = tid - jj * hCi;
+    for (int e0 = tid; e0 < nC; e0 += NE * bt) {
+        T val[NE];
+        T *dst[NE];
+#pragma unroll
+        for (int q = 0; q < NE; ++q) {
+            const int e = e0 + q * bt;
+            const I j = jb + jj;
+            T *cp = p.C + (i64)j * p.ldc + r;
+            dst[q] = nullptr;
+            if (e < nC) {
+                if (r < shift || r > rowlast) {
+                    val[q] = b0 ? T(0) : p.beta * *cp;
+                    dst[q] = cp;
+                } else if (j >= jprod) {  // no product contribution in this column
+                    if (j < jscale) {
+                        val[q] = b0 ? T(0) : p.beta * *cp;
+                        dst[q] = cp;
+                    }
+                } else {
+                    const I i = j - Ctu + r;
+                    if (i >= 0 && i < n) {  // rows outside the matrix are never touched
+                        T acc = b0 ? T(0) : p.beta * *cp;
+                        const I vlo = mm_max(mm_max((I)0, j - Bu), i - Al);
+                        const I vhi = mm_min(mm_min(k - 1, j + Bl), i + Au);
+                        // v ascending: b_vj at sB[jj*SB + (Bu + v - j)], a_iv at sA[(v - v0)*SA + (Au + i - v)]
+                        const T *bp = sB + jj * SBi + (int)(Bu + vlo - j);
+                        const T *ap = sA + (int)(vlo - vb0) * SAi + (int)(Au + i - vlo);
+                        const int cnt = (int)(vhi - vlo) + 1;
+                        int v = 0;
+                        for (; v + 4 <= cnt; v += 4) {  // four steps' LDS reads issued before their fmas
+                            const T x0 = bp[0], x1 = bp[1], x2 = bp[2], x3 = bp[3];
+                            const T a0 = ap[0], a1 = ap[sa1], a2 = ap[2 * sa1], a3 = ap[3 * sa1];
+                            acc = dfma(x0, a0, acc);
+                            acc = dfma(x1, a1, acc);
+                            acc = dfma(x2, a2, acc);
+                            acc = dfma(x3, a3, acc);
+                            bp += 4;
+                            ap += 4 * sa1;
+                        }
+                        for (; v < cnt; ++v) {
+                            acc = dfma(*bp, *ap, acc);
+                            ++bp;
+                            ap += sa1;
+                        }
+                        val[q] = acc;
+                        dst[q] = cp;
+                    }
+                }
+            }
+            r += sr;
+            jj += sj;
+            if (r >= hCi) {
+                r -= hCi;
+                ++jj;
+            }
         }
-        if (j >= p.k + p.Bu) {  // no product contribution in this column
-            if (j < imin(p.m, p.n + p.Cu)) *cp = (p.beta == T(0)) ? T(0) : p.beta * *cp;
-            continue;
-        }
-        const i64 i = j - p.Ctu + r;
-        if (i < 0 || i >= p.n) continue;  // outside the matrix: never touched
-        T acc = (p.beta == T(0)) ? T(0) : p.beta * *cp;
-        const i64 vlo = imax(imax(0, j - p.Bu), i - p.Al), vhi = imin(imin(p.k - 1, j + p.Bl), i + p.Au);
-        // v ascending: b_vj at sB[jj*SB + (Bu + v - j)], a_iv at sA[(v - v0)*SA + (Au + i - v)]
-        const T *bp = sB + jj * SBi + (int)(p.Bu + vlo - j);
-        const T *ap = sA + (int)(vlo - v0) * SAi + (int)(p.Au + i - vlo);
-        const int cnt = (int)(vhi - vlo) + 1, sa1 = SAi - 1;
-        int v = 0;
-        for (; v + 4 <= cnt; v += 4) {  // four steps' LDS reads issued before their fmas
-            const T b0 = bp[0], b1 = bp[1], b2 = bp[2], b3 = bp[3];
-            const T a0 = ap[0], a1 = ap[sa1], a2 = ap[2 * sa1], a3 = ap[3 * sa1];
-            acc = dfma(b0, a0, acc);
-            acc = dfma(b1, a1, acc);
-            acc = dfma(b2, a2, acc);
-            acc = dfma(b3, a3, acc);
-            bp += 4;
-            ap += 4 * sa1;
-        }
-        for (; v < cnt; ++v) {
-            acc = dfma(*bp, *ap, acc);
-            ++bp;
-            ap += sa1;
-        }
-        *cp = acc;
+#pragma unroll
+        for (int q = 0; q < NE; ++q)
+            if (dst[q]) *dst[q] = val[q];
     }
 }
 
@@ -217,12 +278,13 @@ static int gbmm_impl(i64 n, i64 k, i64 m, T alpha, const T *A, i64 lda, i64 Al, 
     }
     GbmmArgs<T> p{n, k, m, alpha, beta, A, lda, Al, Au, B, ldb, Bl, Bu, C, ldc, Ctl, Ctu, Cl, Cu};
     const i64 hA = Al + Au + 1, hB = Bl + Bu + 1;
-    // columns per workgroup: 64 (measured at n = 2^20, (8,8)x(4,4): 3.1 TB/s
-    // against 1.5 TB/s at 256, where 54 KiB of LDS per workgroup left two
-    // workgroups per CU), halved while the launch would have fewer than 512
-    // workgroups, and kept within 64 KiB of LDS.  The column count changes no
-    // result.  BANDEDHIP_GBMM_NC overrides the start (a power of two <= 256).
-    i64 NC = 64;
+    // columns per workgroup: 64 FP64 / 128 FP32 (measured at n = 2^20,
+    // (8,8)x(4,4): FP64 105 us at 64 against 116 us at 128 and 131 us at 32;
+    // FP32 83 us at 128, 86 us at 64), halved while the launch would have
+    // fewer than 512 workgroups, and kept within 64 KiB of LDS.  The column
+    // count changes no result.  BANDEDHIP_GBMM_NC overrides the start (a
+    // power of two <= 256).
+    i64 NC = sizeof(T) == 4 ? 128 : 64;
     while (NC > 8 && cdiv(m, NC) < 512) NC /= 2;
     if (const char *e = getenv("BANDEDHIP_GBMM_NC")) {
         const i64 v = atoll(e);
@@ -235,7 +297,14 @@ static int gbmm_impl(i64 n, i64 k, i64 m, T alpha, const T *A, i64 lda, i64 Al, 
         set_error("gbmm: bandwidths too large for the LDS-staged kernel");
         return -6;
     }
-    hipLaunchKernelGGL(gbmm_kernel<T>, dim3((unsigned)cdiv(m, NC)), dim3(kMmThreads), lds, stream(), p, NC);
+    // 32-bit index arithmetic when every size (and the sums formed from them) fits
+    const i64 big = imax(imax(n, k), imax(m, hC + hA + hB));
+    if (big < (i64(1) << 30))
+        hipLaunchKernelGGL((gbmm_kernel<T, int, kMmBatch>), dim3((unsigned)cdiv(m, NC)), dim3(kMmThreads), lds,
+                           stream(), p, NC);
+    else
+        hipLaunchKernelGGL((gbmm_kernel<T, i64, kMmBatch>), dim3((unsigned)cdiv(m, NC)), dim3(kMmThreads), lds,
+                           stream(), p, NC);
     BH_LAUNCH_CHECK();
     return BH_OK;
 }

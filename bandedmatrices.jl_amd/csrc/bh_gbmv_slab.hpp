@@ -1,5 +1,8 @@
-// bh_gbmv_slab.hip -- y <- alpha*A*x + beta*y ('N') for unit strides and
+// bh_gbmv_slab.hpp -- y <- alpha*A*x + beta*y ('N') for unit strides and
 // contiguous band storage (lda == kl+ku+1), the layout of BandedMatrix.data.
+// The launcher is instantiated once per precision in bh_gbmv_slab_{d,s}.hip,
+// so the two compile in parallel.
+#pragma once
 //
 // A workgroup owns R consecutive rows.  The band columns those rows touch,
 // [r0-kl, r0+R+ku), are one contiguous range of A; it is copied into LDS with
@@ -11,6 +14,8 @@
 // the result is bit-identical to it.  A is read from HBM once (plus the
 // kl+ku columns of halo per R rows); with several right-hand sides the slab
 // is reused for all of them.
+#include <type_traits>
+
 #include "bh_common.hpp"
 
 #include <cstdlib>
@@ -147,66 +152,84 @@ __global__ void __launch_bounds__(NT) gbmv_n_slab(int m, int n, int kl, int ku, 
     const T *slab = reinterpret_cast<const T *>(lds + shift);  // slab[c*h + t]: band row t of column c0+c
     T *ax = reinterpret_cast<T *>(lds + ax_off);
 
-    for (int rb = 0; rb < nrhs; rb += XD) {
-        CFor<XD>::run([&](auto di) {
-            constexpr int d = decltype(di)::value;
-            const int r = rb + d;
+    // One right-hand side: alpha*x to LDS, the x loads XD right-hand sides
+    // ahead, the rows' products, the stores.  The workgroup's interior/edge
+    // choice and the last, partial group of XD are compile-time variants, so
+    // in the steady state every step issues the same memory operations (two
+    // x loads and RPT stores, no early exit) and the compiler's wait for an x
+    // value issued XD steps earlier counts every operation issued since,
+    // instead of draining the whole prefetch.
+    int rb = 0;
+    auto step = [&](auto di, auto interiorc, auto tailc) __attribute__((always_inline)) {
+        constexpr int d = decltype(di)::value;
+        constexpr bool inner = decltype(interiorc)::value, tail = decltype(tailc)::value;
+        const int r = rb + d;
+        if constexpr (tail) {
             if (r >= nrhs) return;  // uniform
-            CFor<XPT>::run([&](auto qi) {
-                constexpr int q = decltype(qi)::value;
-                const int c = tid + q * NT;
-                if (c < nc) ax[c] = alpha * xq[d][q];
-            });
-            __syncthreads();
-            load_x(xq[d], r + XD);
-            if (interior) {
-                // every row has all h band entries inside the matrix: a uniform
-                // trip count, so the LDS reads of eight steps are issued before
-                // their fmas consume them (same ascending-j fma order)
+        }
+        CFor<XPT>::run([&](auto qi) {
+            constexpr int q = decltype(qi)::value;
+            const int c = tid + q * NT;
+            if (c < nc) ax[c] = alpha * xq[d][q];
+        });
+        __syncthreads();
+        load_x(xq[d], r + XD);
+        if constexpr (inner) {
+            // every row has all h band entries inside the matrix: a uniform
+            // trip count, so the LDS reads of eight steps are issued before
+            // their fmas consume them (same ascending-j fma order)
 #pragma unroll
-                for (int q = 0; q < RPT; ++q) {
-                    const int li = tid + q * NT;
-                    T *yp = Y + (i64)r * ldy + r0 + li;
-                    T acc = (beta == T(0)) ? T(0) : beta * *yp;
-                    const T *ap = slab + li * h + (kl + ku);  // entry (i, i-kl+d) at ap[d*(h-1)]
-                    const T *xp = ax + li;                    // alpha*x_{i-kl+d} at xp[d]
-                    int e = 0;
-                    for (; e + 8 <= h; e += 8) {
-                        T av[8], xw[8];
+            for (int q = 0; q < RPT; ++q) {
+                const int li = tid + q * NT;
+                T *yp = Y + (i64)r * ldy + r0 + li;
+                T acc = (beta == T(0)) ? T(0) : beta * *yp;
+                const T *ap = slab + li * h + (kl + ku);  // entry (i, i-kl+d) at ap[d*(h-1)]
+                const T *xp = ax + li;                    // alpha*x_{i-kl+d} at xp[d]
+                int e = 0;
+                for (; e + 8 <= h; e += 8) {
+                    T av[8], xw[8];
 #pragma unroll
-                        for (int u = 0; u < 8; ++u) {
-                            av[u] = ap[(e + u) * (h - 1)];
-                            xw[u] = xp[e + u];
-                        }
-#pragma unroll
-                        for (int u = 0; u < 8; ++u) acc = dfma(xw[u], av[u], acc);
+                    for (int u = 0; u < 8; ++u) {
+                        av[u] = ap[(e + u) * (h - 1)];
+                        xw[u] = xp[e + u];
                     }
-                    for (; e < h; ++e) acc = dfma(xp[e], ap[e * (h - 1)], acc);
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) acc = dfma(xw[u], av[u], acc);
+                }
+                for (; e < h; ++e) acc = dfma(xp[e], ap[e * (h - 1)], acc);
+                *yp = acc;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                const int i = r0 + tid + q * NT;
+                if (i < m) {
+                    T *yp = Y + (i64)r * ldy + i;
+                    T acc = (beta == T(0)) ? T(0) : beta * *yp;
+                    const int jlo = i - kl > 0 ? i - kl : 0;
+                    const int jhi = (i + ku < n - 1) ? i + ku : n - 1;
+                    const T *ap = slab + (jlo - c0) * h + (ku + i - jlo);
+                    const T *xp = ax + (jlo - c0);
+                    for (int j = jlo; j <= jhi; ++j) {
+                        acc = dfma(*xp, *ap, acc);
+                        ++xp;
+                        ap += h - 1;
+                    }
                     *yp = acc;
                 }
-            } else {
-#pragma unroll
-                for (int q = 0; q < RPT; ++q) {
-                    const int i = r0 + tid + q * NT;
-                    if (i < m) {
-                        T *yp = Y + (i64)r * ldy + i;
-                        T acc = (beta == T(0)) ? T(0) : beta * *yp;
-                        const int jlo = i - kl > 0 ? i - kl : 0;
-                        const int jhi = (i + ku < n - 1) ? i + ku : n - 1;
-                        const T *ap = slab + (jlo - c0) * h + (ku + i - jlo);
-                        const T *xp = ax + (jlo - c0);
-                        for (int j = jlo; j <= jhi; ++j) {
-                            acc = dfma(*xp, *ap, acc);
-                            ++xp;
-                            ap += h - 1;
-                        }
-                        *yp = acc;
-                    }
-                }
             }
-            __syncthreads();
-        });
-    }
+        }
+        __syncthreads();
+    };
+    auto all = [&](auto interiorc) __attribute__((always_inline)) {
+        for (; rb + XD <= nrhs; rb += XD)
+            CFor<XD>::run([&](auto di) { step(di, interiorc, std::false_type{}); });
+        if (rb < nrhs) CFor<XD>::run([&](auto di) { step(di, interiorc, std::true_type{}); });
+    };
+    if (interior)
+        all(std::true_type{});
+    else
+        all(std::false_type{});
 }
 
 template <typename T, int NT, int RPT, int NPF>
@@ -507,10 +530,5 @@ int slab_gbmv_n(i64 m, i64 n, i64 kl, i64 ku, i64 nrhs, T alpha, const T *A, i64
     if (r == -1) r = slab_launch<T, 512, 1>(mi, ni, li, ui, ri, alpha, A, ae, X, ldx, beta, Y, ldy, st);
     return r;
 }
-
-template int slab_gbmv_n<double>(i64, i64, i64, i64, i64, double, const double *, i64, const double *, i64, i64,
-                                 double, double *, i64, i64, hipStream_t);
-template int slab_gbmv_n<float>(i64, i64, i64, i64, i64, float, const float *, i64, const float *, i64, i64, float,
-                                float *, i64, i64, hipStream_t);
 
 }  // namespace bh
