@@ -221,13 +221,27 @@ gbmv_t_tile(i64 m, i64 n, i64 kl, i64 ku, i64 nrhs, T alpha, const T *__restrict
     T *xs = slab + W * S;  // x rows [j0-ku, j0+R+kl)
     const i64 x0 = j0 - ku;
 
+    // the first right-hand side's x (the thread's first two entries) is
+    // loaded before the slab, so both wait for memory together
+    T xpre[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const i64 e = threadIdx.x + u * R, row = x0 + e;
+        xpre[u] = (e < W && row >= 0 && row < m) ? X[row * incx] : T(0);
+    }
     load_slab(slab, S, A, lda, j0, c1, h);
     const i64 j = j0 + threadIdx.x;
     // the whole tile is full columns: j0 - ku >= 0, the last column's band ends inside the rows
     const bool interior = j0 - ku >= 0 && j0 + R <= n && j0 + R - 1 + kl <= m - 1;
     for (i64 r = 0; r < nrhs; ++r) {
         const T *x = X + r * ldx;
-        for (i64 e = threadIdx.x; e < W; e += blockDim.x) {
+        i64 e = threadIdx.x;
+        if (r == 0) {
+#pragma unroll
+            for (int u = 0; u < 2; ++u, e += blockDim.x)
+                if (e < W) xs[e] = xpre[u];
+        }
+        for (; e < W; e += blockDim.x) {
             i64 row = x0 + e;
             xs[e] = (row >= 0 && row < m) ? x[row * incx] : T(0);
         }
