@@ -130,6 +130,16 @@ def gpu_time(bh, fn, reps):
     return t.elapsed_ms() / reps
 
 
+def warm(bh, fn, ms=100.0):
+    """Run fn back to back for about ms of GPU time, so the timed loop after
+    it starts at steady clocks (the host-side set-up of a workload's operands
+    leaves the GPU idle long enough to clock down)."""
+    one = gpu_time(bh, fn, 1)
+    for _ in range(int(min(2000, max(1, ms / max(one, 1e-3))))):
+        fn()
+    bh.sync()
+
+
 # ----------------------------------------------------------------- precision (headline)
 def bench_precision(bh, steps, warmup, shift=True):
     import ctypes as C
@@ -230,9 +240,7 @@ def bench_bandwidth(bh, steps, skip_cpu=True):
             k[0] += 1
             bh.mul(y, A, x)
 
-        for _ in range(6):
-            one()
-        bh.sync()
+        warm(bh, one)
         ms = gpu_time(bh, one, max(30, steps * 6))
         nbytes = (17 + 2) * np.dtype(dt).itemsize * float(n)
         out[f"gbmv_{tag}_8x8_n2p20"] = {"GBps": nbytes / ms / 1e6, "frac_of_hbm_peak": nbytes / ms / 1e6 / HBM_PEAK_GBPS,
@@ -244,9 +252,7 @@ def bench_bandwidth(bh, steps, skip_cpu=True):
                 k[0] += 1
                 bh.mul(y, A, x, trans="T")
 
-            for _ in range(6):
-                one_t()
-            bh.sync()
+            warm(bh, one_t)
             ms = gpu_time(bh, one_t, max(30, steps * 6))
             out["gbmv_fp64_T_8x8_n2p20"] = {"GBps": nbytes / ms / 1e6, "frac_of_hbm_peak": nbytes / ms / 1e6 /
                                             HBM_PEAK_GBPS, "us": ms * 1e3, "bytes": nbytes}
@@ -256,9 +262,7 @@ def bench_bandwidth(bh, steps, skip_cpu=True):
         Y = bh.DeviceArray((n, 64), dt)
         A0 = sets[0][0]
         f = lambda: bh.mul(Y, A0, X)
-        for _ in range(3):
-            f()
-        bh.sync()
+        warm(bh, f)
         ms = gpu_time(bh, f, max(20, steps * 4))
         nbytes = (17 + 64 + 64) * float(es) * n
         out[f"gbmv_{tag}_8x8_64rhs_n2p20"] = {"GBps": nbytes / ms / 1e6, "frac_of_hbm_peak": nbytes / ms / 1e6 /
@@ -268,9 +272,7 @@ def bench_bandwidth(bh, steps, skip_cpu=True):
         Bm = bh.BandedMatrix.from_host(np.asfortranarray(rng.standard_normal((9, n)).astype(dt)), n, 4, 4)
         Cm = bh.BandedMatrix.zeros((n, n), (12, 12), dt)
         g = lambda: bh.gbmm(A0, Bm, Cm)
-        for _ in range(3):
-            g()
-        bh.sync()
+        warm(bh, g)
         ms = gpu_time(bh, g, max(40, steps * 8))
         nbytes = (17 + 9 + 25) * float(es) * n
         out[f"gbmm_{tag}_8x8_4x4_n2p20"] = {"GBps": nbytes / ms / 1e6, "frac_of_hbm_peak": nbytes / ms / 1e6 /
