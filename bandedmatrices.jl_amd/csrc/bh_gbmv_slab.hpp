@@ -150,30 +150,40 @@ __global__ void __launch_bounds__(NT) gbmv_n_slab(int m, int n, int kl, int ku, 
         if (k < nchunks) *reinterpret_cast<uint4 *>(lds + 16 * k) = pre[q];
     }
     const T *slab = reinterpret_cast<const T *>(lds + shift);  // slab[c*h + t]: band row t of column c0+c
-    T *ax = reinterpret_cast<T *>(lds + ax_off);
+    T *ax = reinterpret_cast<T *>(lds + ax_off);              // alpha*x of the pair's first right-hand side
+    T *ax2 = ax + (RPT + 1) * NT;                              // and of its second
 
-    // One right-hand side: alpha*x to LDS, the x loads XD right-hand sides
-    // ahead, the rows' products, the stores.  The workgroup's interior/edge
-    // choice and the last, partial group of XD are compile-time variants, so
-    // in the steady state every step issues the same memory operations (two
-    // x loads and RPT stores, no early exit) and the compiler's wait for an x
-    // value issued XD steps earlier counts every operation issued since,
+    // Two right-hand sides per step: alpha*x of both to LDS, their x loads XD
+    // right-hand sides ahead, the rows' products, the stores.  Each band entry
+    // read from the slab serves both products (the slab reads per output drop
+    // from 2h to 1.5h; LDS bandwidth bounds this kernel), and each output
+    // keeps its own ascending fma order.  The workgroup's interior/edge choice
+    // and the last, partial group of XD are compile-time variants, so in the
+    // steady state every step issues the same memory operations (four x loads
+    // and 2 RPT stores, no early exit) and the compiler's wait for an x value
+    // issued XD right-hand sides earlier counts every operation issued since,
     // instead of draining the whole prefetch.
     int rb = 0;
-    auto step = [&](auto di, auto interiorc, auto tailc) __attribute__((always_inline)) {
-        constexpr int d = decltype(di)::value;
+    auto step = [&](auto pi, auto interiorc, auto tailc) __attribute__((always_inline)) {
+        constexpr int d0 = 2 * decltype(pi)::value, d1 = d0 + 1;
         constexpr bool inner = decltype(interiorc)::value, tail = decltype(tailc)::value;
-        const int r = rb + d;
+        const int r = rb + d0;
+        bool two = true;  // the pair's second right-hand side exists (uniform)
         if constexpr (tail) {
             if (r >= nrhs) return;  // uniform
+            two = r + 1 < nrhs;
         }
         CFor<XPT>::run([&](auto qi) {
             constexpr int q = decltype(qi)::value;
             const int c = tid + q * NT;
-            if (c < nc) ax[c] = alpha * xq[d][q];
+            if (c < nc) {
+                ax[c] = alpha * xq[d0][q];
+                ax2[c] = alpha * xq[d1][q];
+            }
         });
         __syncthreads();
-        load_x(xq[d], r + XD);
+        load_x(xq[d0], r + XD);
+        load_x(xq[d1], r + 1 + XD);
         if constexpr (inner) {
             // every row has all h band entries inside the matrix: a uniform
             // trip count, so the LDS reads of eight steps are issued before
@@ -183,21 +193,32 @@ __global__ void __launch_bounds__(NT) gbmv_n_slab(int m, int n, int kl, int ku, 
                 const int li = tid + q * NT;
                 T *yp = Y + (i64)r * ldy + r0 + li;
                 T acc = (beta == T(0)) ? T(0) : beta * *yp;
+                T acc2 = (beta == T(0) || !two) ? T(0) : beta * yp[ldy];
                 const T *ap = slab + li * h + (kl + ku);  // entry (i, i-kl+d) at ap[d*(h-1)]
                 const T *xp = ax + li;                    // alpha*x_{i-kl+d} at xp[d]
+                const T *xp2 = ax2 + li;
                 int e = 0;
                 for (; e + 8 <= h; e += 8) {
-                    T av[8], xw[8];
+                    T av[8], xw[8], xw2[8];
 #pragma unroll
                     for (int u = 0; u < 8; ++u) {
                         av[u] = ap[(e + u) * (h - 1)];
                         xw[u] = xp[e + u];
+                        xw2[u] = xp2[e + u];
                     }
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) acc = dfma(xw[u], av[u], acc);
+                    for (int u = 0; u < 8; ++u) {
+                        acc = dfma(xw[u], av[u], acc);
+                        acc2 = dfma(xw2[u], av[u], acc2);
+                    }
                 }
-                for (; e < h; ++e) acc = dfma(xp[e], ap[e * (h - 1)], acc);
+                for (; e < h; ++e) {
+                    const T a = ap[e * (h - 1)];
+                    acc = dfma(xp[e], a, acc);
+                    acc2 = dfma(xp2[e], a, acc2);
+                }
                 *yp = acc;
+                if (two) yp[ldy] = acc2;
             }
         } else {
 #pragma unroll
@@ -206,25 +227,31 @@ __global__ void __launch_bounds__(NT) gbmv_n_slab(int m, int n, int kl, int ku, 
                 if (i < m) {
                     T *yp = Y + (i64)r * ldy + i;
                     T acc = (beta == T(0)) ? T(0) : beta * *yp;
+                    T acc2 = (beta == T(0) || !two) ? T(0) : beta * yp[ldy];
                     const int jlo = i - kl > 0 ? i - kl : 0;
                     const int jhi = (i + ku < n - 1) ? i + ku : n - 1;
                     const T *ap = slab + (jlo - c0) * h + (ku + i - jlo);
                     const T *xp = ax + (jlo - c0);
+                    const T *xp2 = ax2 + (jlo - c0);
                     for (int j = jlo; j <= jhi; ++j) {
                         acc = dfma(*xp, *ap, acc);
+                        acc2 = dfma(*xp2, *ap, acc2);
                         ++xp;
+                        ++xp2;
                         ap += h - 1;
                     }
                     *yp = acc;
+                    if (two) yp[ldy] = acc2;
                 }
             }
         }
         __syncthreads();
     };
+    static_assert(XD % 2 == 0, "right-hand sides go in pairs");
     auto all = [&](auto interiorc) __attribute__((always_inline)) {
         for (; rb + XD <= nrhs; rb += XD)
-            CFor<XD>::run([&](auto di) { step(di, interiorc, std::false_type{}); });
-        if (rb < nrhs) CFor<XD>::run([&](auto di) { step(di, interiorc, std::true_type{}); });
+            CFor<XD / 2>::run([&](auto pi) { step(pi, interiorc, std::false_type{}); });
+        if (rb < nrhs) CFor<XD / 2>::run([&](auto pi) { step(pi, interiorc, std::true_type{}); });
     };
     if (interior)
         all(std::true_type{});
@@ -236,7 +263,7 @@ template <typename T, int NT, int RPT, int NPF>
 int slab_launch_npf(int m, int n, int kl, int ku, int nrhs, T alpha, const T *A, i64 a_elems, const T *X, i64 ldx,
                     T beta, T *Y, i64 ldy, int slab_cap, hipStream_t st) {
     constexpr int R = NT * RPT;
-    const size_t lds = (size_t)slab_cap + (size_t)(RPT + 1) * NT * sizeof(T);
+    const size_t lds = (size_t)slab_cap + 2 * (size_t)(RPT + 1) * NT * sizeof(T);  // slab + two alpha*x rows
     auto kern = gbmv_n_slab<T, NT, RPT, NPF>;
     if (lds > 64 * 1024)
         BH_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -258,7 +285,7 @@ int slab_launch(int m, int n, int kl, int ku, int nrhs, T alpha, const T *A, i64
     const int h = kl + ku + 1;
     const i64 span = (i64)(R + kl + ku) * h * (i64)sizeof(T) + 32;
     const i64 slab_cap = cdiv(span, 16) * 16;
-    if (slab_cap + (i64)(RPT + 1) * NT * (i64)sizeof(T) > 160 * 1024) return -1;
+    if (slab_cap + 2 * (i64)(RPT + 1) * NT * (i64)sizeof(T) > 160 * 1024) return -1;
     const i64 need = cdiv(span, 16 * (i64)NT);
 #define BH_NPF(P)                                                                                               \
     if (need <= P)                                                                                              \
