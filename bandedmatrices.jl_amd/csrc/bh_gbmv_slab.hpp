@@ -53,6 +53,9 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
     return x < rem ? x * (per + 1) + k : rem * (per + 1) + (x - rem) * per + k;
 }
 
+// right-hand sides per step of gbmv_n_slab
+constexpr int kSlabRP = 4;
+
 template <typename T, int NT, int RPT, int NPF>
 __global__ void __launch_bounds__(NT) gbmv_n_slab(int m, int n, int kl, int ku, int nrhs, T alpha,
                                                   const T *__restrict__ A, i64 a_elems, const T *__restrict__ X,
@@ -117,6 +120,8 @@ __global__ void __launch_bounds__(NT) gbmv_n_slab(int m, int n, int kl, int ku, 
     }
     constexpr int XPT = RPT + 1;  // x values per thread (nc <= R + kl + ku <= XPT * NT when kl+ku <= NT)
     constexpr int XD = 8;         // right-hand sides whose x is in flight
+    constexpr int RP = kSlabRP;   // right-hand sides per step (they share each slab read)
+    constexpr int CH = 8 / (RP > 2 ? 2 : 1);  // band entries whose LDS reads go out before their fmas
     // x of right-hand sides r .. r+XD-1 are loaded into registers XD
     // right-hand sides ahead of their use, so with many right-hand sides the
     // x loads overlap the products instead of waiting once per right-hand side
@@ -150,75 +155,77 @@ __global__ void __launch_bounds__(NT) gbmv_n_slab(int m, int n, int kl, int ku, 
         if (k < nchunks) *reinterpret_cast<uint4 *>(lds + 16 * k) = pre[q];
     }
     const T *slab = reinterpret_cast<const T *>(lds + shift);  // slab[c*h + t]: band row t of column c0+c
-    T *ax = reinterpret_cast<T *>(lds + ax_off);              // alpha*x of the pair's first right-hand side
-    T *ax2 = ax + (RPT + 1) * NT;                              // and of its second
+    // alpha*x of the step's RP right-hand sides: row k at ax + k * (RPT + 1) * NT
+    T *ax = reinterpret_cast<T *>(lds + ax_off);
+    constexpr int AXS = (RPT + 1) * NT;
 
-    // Two right-hand sides per step: alpha*x of both to LDS, their x loads XD
+    // RP right-hand sides per step: alpha*x of each to LDS, their x loads XD
     // right-hand sides ahead, the rows' products, the stores.  Each band entry
-    // read from the slab serves both products (the slab reads per output drop
-    // from 2h to 1.5h; LDS bandwidth bounds this kernel), and each output
-    // keeps its own ascending fma order.  The workgroup's interior/edge choice
-    // and the last, partial group of XD are compile-time variants, so in the
-    // steady state every step issues the same memory operations (four x loads
-    // and 2 RPT stores, no early exit) and the compiler's wait for an x value
-    // issued XD right-hand sides earlier counts every operation issued since,
-    // instead of draining the whole prefetch.
+    // read from the slab serves all RP products (LDS bandwidth bounds this
+    // kernel: the slab reads per output drop from 2h to (1 + 1/RP)h), and
+    // each output keeps its own ascending fma order.  The workgroup's
+    // interior/edge choice and the last, partial group of XD are compile-time
+    // variants, so in the steady state every step issues the same memory
+    // operations (2 RP x loads and RP RPT stores, no early exit) and the
+    // compiler's wait for an x value issued XD right-hand sides earlier counts
+    // every operation issued since, instead of draining the whole prefetch.
     int rb = 0;
     auto step = [&](auto pi, auto interiorc, auto tailc) __attribute__((always_inline)) {
-        constexpr int d0 = 2 * decltype(pi)::value, d1 = d0 + 1;
+        constexpr int d0 = RP * decltype(pi)::value;
         constexpr bool inner = decltype(interiorc)::value, tail = decltype(tailc)::value;
         const int r = rb + d0;
-        bool two = true;  // the pair's second right-hand side exists (uniform)
+        int live = RP;  // right-hand sides of this step that exist (uniform)
         if constexpr (tail) {
             if (r >= nrhs) return;  // uniform
-            two = r + 1 < nrhs;
+            live = nrhs - r < RP ? nrhs - r : RP;
         }
         CFor<XPT>::run([&](auto qi) {
             constexpr int q = decltype(qi)::value;
             const int c = tid + q * NT;
             if (c < nc) {
-                ax[c] = alpha * xq[d0][q];
-                ax2[c] = alpha * xq[d1][q];
+#pragma unroll
+                for (int k = 0; k < RP; ++k) ax[k * AXS + c] = alpha * xq[d0 + k][q];
             }
         });
         __syncthreads();
-        load_x(xq[d0], r + XD);
-        load_x(xq[d1], r + 1 + XD);
+#pragma unroll
+        for (int k = 0; k < RP; ++k) load_x(xq[d0 + k], r + k + XD);
         if constexpr (inner) {
             // every row has all h band entries inside the matrix: a uniform
-            // trip count, so the LDS reads of eight steps are issued before
-            // their fmas consume them (same ascending-j fma order)
+            // trip count, so the LDS reads of CH steps are issued before their
+            // fmas consume them (same ascending-j fma order)
 #pragma unroll
             for (int q = 0; q < RPT; ++q) {
                 const int li = tid + q * NT;
                 T *yp = Y + (i64)r * ldy + r0 + li;
-                T acc = (beta == T(0)) ? T(0) : beta * *yp;
-                T acc2 = (beta == T(0) || !two) ? T(0) : beta * yp[ldy];
-                const T *ap = slab + li * h + (kl + ku);  // entry (i, i-kl+d) at ap[d*(h-1)]
-                const T *xp = ax + li;                    // alpha*x_{i-kl+d} at xp[d]
-                const T *xp2 = ax2 + li;
-                int e = 0;
-                for (; e + 8 <= h; e += 8) {
-                    T av[8], xw[8], xw2[8];
+                T acc[RP];
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) {
+                for (int k = 0; k < RP; ++k) acc[k] = (beta == T(0) || k >= live) ? T(0) : beta * yp[k * ldy];
+                const T *ap = slab + li * h + (kl + ku);  // entry (i, i-kl+d) at ap[d*(h-1)]
+                const T *xp = ax + li;                    // alpha*x_{i-kl+d} of step row k at xp[k*AXS + d]
+                int e = 0;
+                for (; e + CH <= h; e += CH) {
+                    T av[CH], xw[RP][CH];
+#pragma unroll
+                    for (int u = 0; u < CH; ++u) {
                         av[u] = ap[(e + u) * (h - 1)];
-                        xw[u] = xp[e + u];
-                        xw2[u] = xp2[e + u];
+#pragma unroll
+                        for (int k = 0; k < RP; ++k) xw[k][u] = xp[k * AXS + e + u];
                     }
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        acc = dfma(xw[u], av[u], acc);
-                        acc2 = dfma(xw2[u], av[u], acc2);
+                    for (int u = 0; u < CH; ++u) {
+#pragma unroll
+                        for (int k = 0; k < RP; ++k) acc[k] = dfma(xw[k][u], av[u], acc[k]);
                     }
                 }
                 for (; e < h; ++e) {
                     const T a = ap[e * (h - 1)];
-                    acc = dfma(xp[e], a, acc);
-                    acc2 = dfma(xp2[e], a, acc2);
+#pragma unroll
+                    for (int k = 0; k < RP; ++k) acc[k] = dfma(xp[k * AXS + e], a, acc[k]);
                 }
-                *yp = acc;
-                if (two) yp[ldy] = acc2;
+#pragma unroll
+                for (int k = 0; k < RP; ++k)
+                    if (k < live) yp[k * ldy] = acc[k];
             }
         } else {
 #pragma unroll
@@ -226,32 +233,32 @@ __global__ void __launch_bounds__(NT) gbmv_n_slab(int m, int n, int kl, int ku, 
                 const int i = r0 + tid + q * NT;
                 if (i < m) {
                     T *yp = Y + (i64)r * ldy + i;
-                    T acc = (beta == T(0)) ? T(0) : beta * *yp;
-                    T acc2 = (beta == T(0) || !two) ? T(0) : beta * yp[ldy];
+                    T acc[RP];
+#pragma unroll
+                    for (int k = 0; k < RP; ++k) acc[k] = (beta == T(0) || k >= live) ? T(0) : beta * yp[k * ldy];
                     const int jlo = i - kl > 0 ? i - kl : 0;
                     const int jhi = (i + ku < n - 1) ? i + ku : n - 1;
                     const T *ap = slab + (jlo - c0) * h + (ku + i - jlo);
                     const T *xp = ax + (jlo - c0);
-                    const T *xp2 = ax2 + (jlo - c0);
                     for (int j = jlo; j <= jhi; ++j) {
-                        acc = dfma(*xp, *ap, acc);
-                        acc2 = dfma(*xp2, *ap, acc2);
+#pragma unroll
+                        for (int k = 0; k < RP; ++k) acc[k] = dfma(xp[k * AXS], *ap, acc[k]);
                         ++xp;
-                        ++xp2;
                         ap += h - 1;
                     }
-                    *yp = acc;
-                    if (two) yp[ldy] = acc2;
+#pragma unroll
+                    for (int k = 0; k < RP; ++k)
+                        if (k < live) yp[k * ldy] = acc[k];
                 }
             }
         }
         __syncthreads();
     };
-    static_assert(XD % 2 == 0, "right-hand sides go in pairs");
+    static_assert(XD % RP == 0, "a step's right-hand sides divide the prefetch depth");
     auto all = [&](auto interiorc) __attribute__((always_inline)) {
         for (; rb + XD <= nrhs; rb += XD)
-            CFor<XD / 2>::run([&](auto pi) { step(pi, interiorc, std::false_type{}); });
-        if (rb < nrhs) CFor<XD / 2>::run([&](auto pi) { step(pi, interiorc, std::true_type{}); });
+            CFor<XD / RP>::run([&](auto pi) { step(pi, interiorc, std::false_type{}); });
+        if (rb < nrhs) CFor<XD / RP>::run([&](auto pi) { step(pi, interiorc, std::true_type{}); });
     };
     if (interior)
         all(std::true_type{});
@@ -263,7 +270,7 @@ template <typename T, int NT, int RPT, int NPF>
 int slab_launch_npf(int m, int n, int kl, int ku, int nrhs, T alpha, const T *A, i64 a_elems, const T *X, i64 ldx,
                     T beta, T *Y, i64 ldy, int slab_cap, hipStream_t st) {
     constexpr int R = NT * RPT;
-    const size_t lds = (size_t)slab_cap + 2 * (size_t)(RPT + 1) * NT * sizeof(T);  // slab + two alpha*x rows
+    const size_t lds = (size_t)slab_cap + kSlabRP * (size_t)(RPT + 1) * NT * sizeof(T);  // slab + alpha*x rows
     auto kern = gbmv_n_slab<T, NT, RPT, NPF>;
     if (lds > 64 * 1024)
         BH_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
@@ -285,7 +292,7 @@ int slab_launch(int m, int n, int kl, int ku, int nrhs, T alpha, const T *A, i64
     const int h = kl + ku + 1;
     const i64 span = (i64)(R + kl + ku) * h * (i64)sizeof(T) + 32;
     const i64 slab_cap = cdiv(span, 16) * 16;
-    if (slab_cap + 2 * (i64)(RPT + 1) * NT * (i64)sizeof(T) > 160 * 1024) return -1;
+    if (slab_cap + kSlabRP * (i64)(RPT + 1) * NT * (i64)sizeof(T) > 160 * 1024) return -1;
     const i64 need = cdiv(span, 16 * (i64)NT);
 #define BH_NPF(P)                                                                                               \
     if (need <= P)                                                                                              \
