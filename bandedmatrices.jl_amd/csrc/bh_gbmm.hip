@@ -115,6 +115,64 @@ __global__ void __launch_bounds__(kMmThreads) gbmm_kernel(GbmmArgs<T> p, i64 NC)
     const bool b0 = p.beta == T(0);
     const int hCi = (int)hC, nC = (int)((j1 - j0) * hC);
     const int sj = bt / hCi, sr = bt - sj * hCi, sa1 = SAi - 1;
+    // Interior workgroups (every column's product band lies inside B's
+    // columns, inside A's rows and inside the matrix): thread (g, r) keeps
+    // stored row r and takes columns g, g + G, ...  Row r's band offsets,
+    // trip count and slab steps are then the same for all its columns:
+    // v runs over j + dlo .. j + dhi, b_vj at sB[jj*SB + Bu + dlo + t],
+    // a_iv at sA[(jj + Bu + dlo + t)*SA + Au + r - Ctu - dlo - t].
+    const bool inner_wg = j0 >= p.Bu && j0 >= p.Cu && j1 - 1 + p.Bl <= p.k - 1 && j1 - 1 + p.Cl <= p.n - 1 &&
+                          (i64)hC * 2 <= bt;
+    if (inner_wg) {
+        const int G = bt / hCi, g = tid / hCi, rr = tid - g * hCi;
+        if (g >= G) return;  // no LDS use or barrier follows
+        const int ncol = (int)(j1 - j0);
+        const int rc = rr - (int)Ctu;  // row r's offset from the diagonal
+        const bool prod = !(rr < (int)shift || rr > (int)rowlast);
+        const int dlo = rc - (int)Al > -(int)Bu ? rc - (int)Al : -(int)Bu;
+        const int dhi = rc + (int)Au < (int)Bl ? rc + (int)Au : (int)Bl;
+        const int cnt = dhi - dlo + 1;
+        const int boff = (int)Bu + dlo, aoff = ((int)Bu + dlo) * SAi + (int)Au + rc - dlo;
+        T *cbase = p.C + j0 * p.ldc + rr;
+        for (int c0 = g; c0 < ncol; c0 += NE * G) {
+            T val[NE];
+#pragma unroll
+            for (int q = 0; q < NE; ++q) {
+                const int jj2 = c0 + q * G;
+                if (jj2 < ncol) {
+                    T *cp = cbase + (i64)jj2 * p.ldc;
+                    T acc = b0 ? T(0) : p.beta * *cp;
+                    if (prod) {
+                        const T *bp = sB + jj2 * SBi + boff;
+                        const T *ap = sA + jj2 * SAi + aoff;
+                        int v = 0;
+                        for (; v + 4 <= cnt; v += 4) {  // four steps' LDS reads issued before their fmas
+                            const T x0 = bp[0], x1 = bp[1], x2 = bp[2], x3 = bp[3];
+                            const T a0 = ap[0], a1 = ap[sa1], a2 = ap[2 * sa1], a3 = ap[3 * sa1];
+                            acc = dfma(x0, a0, acc);
+                            acc = dfma(x1, a1, acc);
+                            acc = dfma(x2, a2, acc);
+                            acc = dfma(x3, a3, acc);
+                            bp += 4;
+                            ap += 4 * sa1;
+                        }
+                        for (; v < cnt; ++v) {
+                            acc = dfma(*bp, *ap, acc);
+                            ++bp;
+                            ap += sa1;
+                        }
+                    }
+                    val[q] = acc;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NE; ++q) {
+                const int jj2 = c0 + q * G;
+                if (jj2 < ncol) cbase[(i64)jj2 * p.ldc] = val[q];
+            }
+        }
+        return;
+    }
     int jj = tid / hCi, r = tid - jj * hCi;
     for (int e0 = tid; e0 < nC; e0 += NE * bt) {
         T val[NE];
