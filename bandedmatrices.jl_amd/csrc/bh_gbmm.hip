@@ -20,9 +20,9 @@
 namespace bh {
 
 constexpr int kMmThreads = 256;
-// entries per thread computed before their stores issue (measured at n = 2^20,
-// (8,8) x (4,4): 105 us FP64 at 4, 115 us at 8, 108 us at 2, 135 us storing
-// each entry at once)
+// entries per thread computed before their stores issue (measured on the
+// general path at n = 2^20, (8,8) x (4,4): 105 us FP64 at 4, 115 us at 8,
+// 108 us at 2, 135 us storing each entry at once)
 constexpr int kMmBatch = 4;
 
 template <typename T>
@@ -337,11 +337,11 @@ static int gbmm_impl(i64 n, i64 k, i64 m, T alpha, const T *A, i64 lda, i64 Al, 
     GbmmArgs<T> p{n, k, m, alpha, beta, A, lda, Al, Au, B, ldb, Bl, Bu, C, ldc, Ctl, Ctu, Cl, Cu};
     const i64 hA = Al + Au + 1, hB = Bl + Bu + 1;
     // columns per workgroup: 64 FP64 / 128 FP32 (measured at n = 2^20,
-    // (8,8)x(4,4): FP64 105 us at 64 against 116 us at 128 and 131 us at 32;
-    // FP32 83 us at 128, 86 us at 64), halved while the launch would have
-    // fewer than 512 workgroups, and kept within 64 KiB of LDS.  The column
-    // count changes no result.  BANDEDHIP_GBMM_NC overrides the start (a
-    // power of two <= 256).
+    // (8,8)x(4,4), warm: FP64 92 us at 64 against 101 us at 128, 124 us at 32
+    // and 155 us at 256; FP32 65 us at 128, 72 us at 64), halved while the
+    // launch would have fewer than 512 workgroups, and kept within 64 KiB of
+    // LDS.  The column count changes no result.  BANDEDHIP_GBMM_NC overrides
+    // the start (a power of two <= 256).
     i64 NC = sizeof(T) == 4 ? 128 : 64;
     while (NC > 8 && cdiv(m, NC) < 512) NC /= 2;
     if (const char *e = getenv("BANDEDHIP_GBMM_NC")) {
