@@ -90,6 +90,38 @@ def tbsv_(uplo: str, trans: str, diag: str, m: int, k: int, A, x: DeviceArray) -
     return _tb_call("tbsv", uplo, trans, diag, m, k, A, x)
 
 
+_IDENTITY = {}
+
+
+def _identity_pivots(n: int) -> DeviceArray:
+    """1..n as device int64 (no interchanges), kept per length."""
+    ipiv = _IDENTITY.get(n)
+    if ipiv is None:
+        ipiv = DeviceArray.from_numpy(np.arange(1, n + 1, dtype=np.int64))
+        _IDENTITY.clear()
+        _IDENTITY[n] = ipiv
+    return ipiv
+
+
+def _tbsv_partitioned(trans: str, k: int, V: BandView, x: DeviceArray) -> bool:
+    """Upper, non-unit op(T) x = b by the partitioned band solve: T's band data
+    (diagonal at row k, leading dimension >= k+1) is LAPACK's AB layout of a
+    band LU with kl = 0 and ku = k, so bh_?gbtrs_part with identity pivots
+    solves it (rows split into blocks solved at once; equal to tbsv to
+    rounding).  False when k is outside its range (k <= 63)."""
+    from .lu import _part_workspace
+
+    n = V.ncols
+    nrhs = 1 if len(x.shape) == 1 else x.shape[1]
+    p = _prefix(x.dtype)
+    ws = _part_workspace(p, n, 0, k, nrhs)
+    if ws is None:
+        return False
+    _lib.call(f"bh_{p}gbtrs_part", trans.encode(), n, 0, k, nrhs, V.ptr, V.ld, _identity_pivots(n).ptr, x.ptr,
+              x.ld, ws.ptr, ws.nbytes)
+    return True
+
+
 def tbmv_(uplo: str, trans: str, diag: str, m: int, k: int, A, x: DeviceArray) -> DeviceArray:
     """tbmv!(uplo, trans, diag, m, k, A, x): x <- op(T) x in place (src/blas.jl:71-107)."""
     return _tb_call("tbmv", uplo, trans, diag, m, k, A, x)
@@ -130,12 +162,27 @@ class _Triangular:
         D = self.parent.bandeddata()
         return BandView(D, 0, u + 1) if self.UPLO == "U" else BandView(D, u, l + 1)
 
-    def ldiv_(self, x: DeviceArray, trans: str = "N") -> DeviceArray:
-        """ldiv!(T, x) ('N'), ldiv!(transpose(T), x) ('T'), in place."""
+    def ldiv_(self, x: DeviceArray, trans: str = "N", method: str = "exact") -> DeviceArray:
+        """ldiv!(T, x) ('N'), ldiv!(transpose(T), x) ('T'), in place.
+
+        method="exact": tbsv with OpenBLAS's operation order, bit for bit.
+        method="partitioned": upper, non-unit triangles with k <= 63
+        off-diagonals go through the partitioned band solve (equal to
+        rounding); every other case falls back to the exact solve."""
         m, n = self.shape
         if m != n:
             raise DimensionMismatch(f"matrix is not square: dimensions are {m}, {n}")
-        return tbsv_(self.UPLO, trans, self.UNIT, m, self.bandwidth(), self.bandeddata(), x)
+        if method not in ("exact", "partitioned"):
+            raise ValueError(f"unknown solve method {method!r}")
+        k, V = self.bandwidth(), self.bandeddata()
+        if method == "partitioned" and self.UPLO == "U" and self.UNIT == "N" and m > 0:
+            if x.shape[0] != n:
+                raise DimensionMismatch(f"size of A is {n} != length(x) = {x.shape[0]}")
+            if V.dtype != x.dtype:
+                raise TypeError("A and x must have the same element type")
+            if _tbsv_partitioned(trans, k, V, x):
+                return x
+        return tbsv_(self.UPLO, trans, self.UNIT, m, k, V, x)
 
     def lmul_(self, x: DeviceArray, trans: str = "N") -> DeviceArray:
         """lmul!(T, x) ('N'), lmul!(transpose(T), x) ('T'), in place."""

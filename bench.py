@@ -585,6 +585,19 @@ def bench_triangular(bh, steps, skip_cpu):
             r[f"us_{tag}"] = gpu_time(bh, f, max(2, steps)) * 1e3
             X.upload(x_h)
         r["ns_per_step_tbsv_UN"] = r["us_tbsv_UN"] * 1e3 / n
+        if k <= 63:
+            # the upper triangle through the partitioned band solve (kl = 0), to rounding
+            T = bh.UpperTriangular(bh.BandedMatrix.from_host(a_h, n, 0, k))
+            for trans in ("N", "T"):
+                X.upload(x_h)
+                xe = T.ldiv_(X, trans=trans).numpy()
+                X.upload(x_h)
+                xp = T.ldiv_(X, trans=trans, method="partitioned").numpy()
+                r[f"rel_diff_partitioned_vs_exact_U{trans}"] = float(np.abs(xp - xe).max() / np.abs(xe).max())
+                fp = lambda: T.ldiv_(X, trans=trans, method="partitioned")
+                r[f"us_tbsv_U{trans}_partitioned"] = gpu_time(bh, fp, max(5, steps)) * 1e3
+            X.upload(x_h)
+            del T
         XB = bh.DeviceArray.from_numpy(np.tile(x_h[:, None], (1, 64)))
         fb = lambda: bh.tbsv_("U", "N", "N", n, k, A, XB)
         fb()

@@ -280,6 +280,57 @@ def test_gpu_triangular_views_of_banded_matrix(bh, oracle, cls):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n,l,u,nrhs", [(20000, 3, 8, 1), (5000, 0, 40, 3), (3000, 2, 63, 1), (100, 1, 5, 2),
+                                        (1, 0, 0, 1)])
+def test_gpu_triangular_partitioned_solve(bh, oracle, dtype, n, l, u, nrhs):
+    """ldiv_(UpperTriangular(A), x, method="partitioned") -- the partitioned
+    band solve with kl = 0 and no interchanges -- equals tbsv to rounding for
+    'N' and 'T' (max-norm relative 1e-12 FP64 / 1e-4 FP32 against the oracle's
+    tbsv on a well-conditioned triangle); the sub-diagonal rows of the parent
+    (NaN here) are never read; lower and unit-diagonal triangles and k > 63
+    fall back to the exact tbsv (bit for bit)."""
+    rng = np.random.default_rng(n + u)
+    D = np.asfortranarray(rng.uniform(-0.5 / max(u, 1), 0.5 / max(u, 1), size=(l + u + 1, n)))
+    D[u, :] = 1.0 + rng.random(n)
+    D[u + 1:, :] = np.nan  # the lower triangle must not be touched
+    Dd = D.astype(dtype)
+    A = bh.BandedMatrix.from_host(Dd, n, l, u)
+    T = bh.UpperTriangular(A)
+    tri = np.asfortranarray(Dd[: u + 1].astype(np.float64))
+    B = rng.standard_normal((n, nrhs)).astype(dtype)
+    tol = 1e-12 if dtype == np.float64 else 1e-4
+    for trans in ("N", "T"):
+        xb = bh.DeviceArray.from_numpy(np.asfortranarray(B if nrhs > 1 else B[:, 0]))
+        got = T.ldiv_(xb, trans=trans, method="partitioned").numpy().reshape(n, nrhs).astype(np.float64)
+        for c in range(nrhs):
+            ref = oracle.tbsv("U", trans, "N", n, u, tri, B[:, c].astype(np.float64))
+            assert np.abs(got[:, c] - ref).max() <= tol * np.abs(ref).max(), (trans, c)
+    # fallbacks: bit for bit with the exact solve
+    L = bh.LowerTriangular(bh.BandedMatrix.from_host(np.nan_to_num(Dd), n, l, u))
+    x = rng.standard_normal(n).astype(dtype)
+    a = L.ldiv_(bh.DeviceArray.from_numpy(x), method="partitioned").numpy()
+    b = L.ldiv_(bh.DeviceArray.from_numpy(x)).numpy()
+    assert bits_equal(a, b)
+    with pytest.raises(ValueError):
+        T.ldiv_(bh.DeviceArray.from_numpy(x), method="nope")
+
+
+@pytest.mark.gpu
+def test_gpu_triangular_partitioned_wide_band_falls_back(bh):
+    """k = 64 is past the partitioned solve's range: the exact tbsv runs."""
+    rng = np.random.default_rng(3)
+    n, u = 3000, 64
+    D = np.asfortranarray(rng.uniform(-0.01, 0.01, size=(u + 1, n)))
+    D[u, :] = 1.0 + rng.random(n)
+    T = bh.UpperTriangular(bh.BandedMatrix.from_host(D, n, 0, u))
+    x = rng.standard_normal(n)
+    a = T.ldiv_(bh.DeviceArray.from_numpy(x), method="partitioned").numpy()
+    b = T.ldiv_(bh.DeviceArray.from_numpy(x)).numpy()
+    assert bits_equal(a, b)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("uplo,trans,diag", list(itertools.product(UPLO, TRANS, DIAG)))
 def test_gpu_tbsv_multi_equals_single(bh, oracle, uplo, trans, diag):
     """All columns in one launch: each equals the single-vector call bit for bit
