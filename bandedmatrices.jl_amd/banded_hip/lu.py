@@ -172,10 +172,12 @@ def ldiv_(F, B: DeviceArray, method: str = "exact") -> DeviceArray:
     from .symbanded import BandedCholesky, BandedLDLt, cholesky_ldiv_, ldlt_ldiv_
     from .tribanded import _Triangular
 
+    # triangular views and Cholesky factors: "partitioned" as asked, every
+    # other method their exact solve
     if isinstance(F, _Triangular):
-        return F.ldiv_(B)
+        return F.ldiv_(B, method="partitioned" if method == "partitioned" else "exact")
     if isinstance(F, BandedCholesky):
-        return cholesky_ldiv_(F, B)
+        return cholesky_ldiv_(F, B, method="partitioned" if method == "partitioned" else "exact")
     if isinstance(F, BandedLDLt):
         return ldlt_ldiv_(F, B)
     trans = "N"

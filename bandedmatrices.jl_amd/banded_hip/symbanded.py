@@ -163,12 +163,32 @@ def cholesky(S: Symmetric, check: bool = True) -> BandedCholesky:
     return BandedCholesky(F, S.uplo, int(info.value))
 
 
-def cholesky_ldiv_(F: BandedCholesky, B: DeviceArray) -> DeviceArray:
-    """ldiv!(F, B) in place (pbtrs)."""
+def cholesky_ldiv_(F: BandedCholesky, B: DeviceArray, method: str = "exact") -> DeviceArray:
+    """ldiv!(F, B) in place (pbtrs).
+
+    method="exact": LAPACK's pbtrs operation order.  method="partitioned": an
+    upper factor U (its (k+1) x n band is LAPACK's AB layout of a band LU with
+    kl = 0, ku = k) with k <= 63 is solved as U^T y = B, U x = y by the
+    partitioned band solve (bh_?gbtrs_part, identity pivots; equal to rounding);
+    a lower factor or a wider band falls back to the exact solve."""
     n = F.shape[0]
     if B.shape[0] != n:
         raise DimensionMismatch(f"matrix A has dimensions {F.shape}, but right hand side B has {B.shape}")
+    if method not in ("exact", "partitioned"):
+        raise ValueError(f"unknown solve method {method!r}")
     nrhs = 1 if len(B.shape) == 1 else B.shape[1]
+    if method == "partitioned" and F.uplo == "U" and n > 0 and nrhs > 0:
+        from .lu import _part_workspace
+        from .tribanded import _identity_pivots
+
+        p = _prefix(B.dtype)
+        ws = _part_workspace(p, n, 0, F.kd(), nrhs)
+        if ws is not None:
+            ipiv = _identity_pivots(n)
+            for trans in ("T", "N"):
+                _lib.call(f"bh_{p}gbtrs_part", trans.encode(), n, 0, F.kd(), nrhs, F.factors.data.ptr,
+                          F.factors.data.ld, ipiv.ptr, B.ptr, B.ld, ws.ptr, ws.nbytes)
+            return B
     _lib.call(f"bh_{_prefix(B.dtype)}pbtrs", F.uplo.encode(), n, F.kd(), nrhs, F.factors.data.ptr,
               F.factors.data.ld, B.ptr, max(1, B.ld if len(B.shape) == 2 else n))
     return B

@@ -716,6 +716,25 @@ def bench_cholesky(bh, steps, skip_cpu):
         f_rs = lambda: bh._lib.call("bh_dpbtrs", b"U", n, kd, 1, A.ptr, A.ld, X.ptr, n)
         f_rs()
         r["ms_pbtrs"] = gpu_time(bh, f_rs, max(2, steps // 2))
+        if kd <= 63:
+            # U^T y = b, U x = y by the partitioned band solve (kl = 0, identity pivots), to rounding
+            nb = C.c_size_t()
+            bh._lib.call("bh_gbtrs_part_workspace", b"d", n, 0, kd, 1, C.byref(nb))
+            ws = bh.DeviceArray((nb.value,), np.uint8)
+            ip = bh.DeviceArray.from_numpy(np.arange(1, n + 1, dtype=np.int64))
+
+            def f_part():
+                for tr in (b"T", b"N"):
+                    bh._lib.call("bh_dgbtrs_part", tr, n, 0, kd, 1, A.ptr, A.ld, ip.ptr, X.ptr, n, ws.ptr, nb.value)
+
+            X.upload(b)
+            f_rs()
+            xe = X.numpy()
+            X.upload(b)
+            f_part()
+            r["rel_diff_pbtrs_partitioned_vs_exact"] = float(np.abs(X.numpy() - xe).max() / np.abs(xe).max())
+            r["ms_pbtrs_partitioned"] = gpu_time(bh, f_part, max(5, steps))
+            del ws, ip
         Y = bh.DeviceArray((n,), np.float64)
         f_mv = lambda: bh._lib.call("bh_dsbmv", b"U", n, kd, 1.0, A.ptr, A.ld, X.ptr, 1, 0.0, Y.ptr, 1)
         f_mv()

@@ -245,6 +245,30 @@ def test_gpu_cholesky_solve(bh, oracle, uplo):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("n,kd,nrhs", [(20000, 8, 1), (6000, 40, 3), (500, 63, 2)])
+def test_gpu_cholesky_partitioned_solve(bh, oracle, n, kd, nrhs):
+    """ldiv_(cholesky(Symmetric(A)), B, method="partitioned"): an upper factor
+    U solved as U^T y = B, U x = y by the partitioned band solve (kl = 0),
+    within 1e-12 (max norm, relative) of the oracle's pbtrs on the same
+    factor; a lower factor falls back to the exact pbtrs, bit for bit."""
+    rng = np.random.default_rng(n + kd)
+    D = np.asfortranarray(rng.uniform(-1, 1, size=(2 * kd + 1, n)))
+    D[kd, :] = 2.0 * kd + 1.0 + rng.random(n)
+    A = bh.BandedMatrix.from_host(D, n, kd, kd)
+    F = bh.cholesky(bh.Symmetric(A, "U"))
+    f = F.factors.host_data()
+    B = rng.standard_normal((n, nrhs))
+    X = bh.ldiv_(F, bh.DeviceArray.from_numpy(np.asfortranarray(B)), method="partitioned").numpy().reshape(n, nrhs)
+    for c in range(nrhs):
+        xo = oracle.pbtrs("U", n, kd, f, B[:, c])
+        assert np.abs(X[:, c] - xo).max() <= 1e-12 * np.abs(xo).max(), c
+    FL = bh.cholesky(bh.Symmetric(A, "L"))
+    a = bh.ldiv_(FL, bh.DeviceArray.from_numpy(B[:, 0].copy()), method="partitioned").numpy()
+    b = bh.ldiv_(FL, bh.DeviceArray.from_numpy(B[:, 0].copy())).numpy()
+    assert bits_equal(a, b)
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("uplo", ["U", "L"])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 def test_gpu_sbmv_bitexact(bh, oracle, uplo, dtype):
