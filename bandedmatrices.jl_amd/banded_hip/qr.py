@@ -110,8 +110,14 @@ def qr_lmul_(F: BandedQR, B: DeviceArray, trans: str) -> DeviceArray:
     return B
 
 
-def qr_ldiv_(F: BandedQR, B: DeviceArray) -> DeviceArray:
-    """ldiv!(F, B): B (m x nrhs) <- Q'B, then its first n rows <- R \\ (Q'B)[1:n]."""
+def qr_ldiv_(F: BandedQR, B: DeviceArray, method: str = "exact") -> DeviceArray:
+    """ldiv!(F, B): B (m x nrhs) <- Q'B, then its first n rows <- R \\ (Q'B)[1:n].
+
+    method="exact": the reference's order (reflectors, then tbsv with R).
+    method="partitioned": the same Q'B, then R's solve by the partitioned band
+    solve (R's band rows are LAPACK's AB layout of a band LU with kl = 0 and
+    ku = R's bandwidth; identity pivots), equal to rounding, for R bandwidths
+    <= 63; wider R falls back to the exact solve."""
     m, n = F.shape
     if m < n:
         raise ValueError("Not implemented for wide matrices")  # bandedqr.jl:175-177
@@ -119,15 +125,28 @@ def qr_ldiv_(F: BandedQR, B: DeviceArray) -> DeviceArray:
         raise DimensionMismatch(f"B has {B.shape[0]} rows, A has {m}")
     if B.dtype != F.factors.dtype:
         raise TypeError("mixed element types")
+    if method not in ("exact", "partitioned"):
+        raise ValueError(f"unknown solve method {method!r}")
     nrhs = 1 if len(B.shape) == 1 else B.shape[1]
     l, u = F.factors.bandwidths()
+    if method == "partitioned" and n > 0 and nrhs > 0:
+        from .lu import _part_workspace
+        from .tribanded import _identity_pivots
+
+        p = _prefix(B.dtype)
+        ws = _part_workspace(p, n, 0, u, nrhs)
+        if ws is not None:
+            qr_lmul_(F, B, "T")
+            _lib.call(f"bh_{p}gbtrs_part", b"N", n, 0, u, nrhs, F.factors.data.ptr, F.factors.data.ld,
+                      _identity_pivots(n).ptr, B.ptr, B.ld, ws.ptr, ws.nbytes)
+            return B
     _lib.call(f"bh_{_prefix(B.dtype)}gbqrs", m, n, l, u, F.factors.data.ptr, F.factors.data.ld, F.tau.ptr, nrhs,
               B.ptr, B.ld)
     return B
 
 
-def qr_solve(A: BandedMatrix, b: DeviceArray) -> DeviceArray:
+def qr_solve(A: BandedMatrix, b: DeviceArray, method: str = "exact") -> DeviceArray:
     """qr(A) \\ b for m >= n: the least-squares solution (first n rows of the result)."""
     F = qr(A)
-    x = qr_ldiv_(F, b.copy())
+    x = qr_ldiv_(F, b.copy(), method)
     return x

@@ -473,6 +473,12 @@ def bench_qr(bh, steps, skip_cpu):
         r["ms_qr"] = gpu_time(bh, lambda: bh.qr(A), max(2, steps // 2))
         r["ms_qr_ldiv"] = gpu_time(bh, lambda: bh.qr_ldiv_(F, b.copy()), max(2, steps // 2))
         x = bh.qr_ldiv_(F, b.copy()).numpy()
+        if l + u <= 63:
+            # Q'b by the reflectors, then R's solve partitioned across the GPU (to rounding)
+            r["ms_qr_ldiv_partitioned"] = gpu_time(bh, lambda: bh.qr_ldiv_(F, b.copy(), method="partitioned"),
+                                                   max(2, steps // 2))
+            xp = bh.qr_ldiv_(F, b.copy(), method="partitioned").numpy()
+            r["rel_diff_qr_ldiv_partitioned_vs_exact"] = float(np.abs(xp - x).max() / np.abs(x).max())
         import scipy.linalg.blas as sb
 
         res = b_h - sb.dgbmv(n, n, l, u, 1.0, A_h, x)

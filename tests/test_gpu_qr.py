@@ -119,6 +119,29 @@ def test_qr_solve(bh, oracle, m, n, l, u):
     assert np.abs(Ad.T @ r).max() <= 1e-9 * np.abs(Ad).sum(axis=0).max() * max(1.0, np.abs(X).max())
 
 
+@pytest.mark.parametrize("m,n,l,u", [(20000, 20000, 3, 4), (400, 300, 3, 5), (700, 700, 31, 20), (500, 500, 6, 0),
+                                     (300, 300, 40, 30)])
+def test_qr_solve_partitioned(bh, oracle, m, n, l, u):
+    """qr_ldiv_(F, B, method="partitioned"): Q'B by the reflectors, then R's
+    solve by the partitioned band solve (kl = 0, R's bandwidth l+u <= 63);
+    within 1e-12 of the oracle's gbqrs on the GPU's factors.  R wider than 63
+    (the last case, l+u = 70) falls back to the exact solve, bit for bit."""
+    rng = np.random.default_rng(5 * m + n + l + u)
+    A = _make(rng, m, n, l, u)
+    A[u, :] += 3.0 * (l + u + 1)
+    B = np.asfortranarray(rng.standard_normal((m, 2)))
+    F = bh.qr(bh.BandedMatrix.from_host(A, m, l, u))
+    X = bh.qr_ldiv_(F, bh.DeviceArray.from_numpy(B), method="partitioned").numpy()[:n]
+    D, tau = F.factors.host_data(), F.tau.numpy()[: min(m, n)]
+    Xo = oracle.gbqrs(m, n, l, l + u, D, tau, B)[:n]
+    assert np.abs(X - Xo).max() <= 1e-12 * max(1.0, np.abs(Xo).max())
+    if l + u > 63:
+        Xe = bh.qr_ldiv_(F, bh.DeviceArray.from_numpy(B)).numpy()[:n]
+        assert bits_equal(X, Xe)
+    with pytest.raises(ValueError):
+        bh.qr_ldiv_(F, bh.DeviceArray.from_numpy(B), method="nope")
+
+
 def test_qr_fp32_close_to_fp64(bh, oracle):
     m = n = 2000
     l, u = 4, 3
