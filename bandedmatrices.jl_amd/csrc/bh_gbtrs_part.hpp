@@ -523,8 +523,16 @@ int part_solve(char trans, i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 lda
     a.ldb = ldb;
     a.yout = Y;
     a.ldy = n;
-    int stt;
-    if (!tr) {
+    int stt = BH_OK;
+    // kl = 0 (a triangular U): the L sweeps have no multipliers and no
+    // interchanges (an interchange stays within rows j .. j + kl, as LAPACK's
+    // xGBTRS skips L when kl = 0), so they run as plain copies and leave their
+    // growth word at zero
+    const bool copy_l = kl == 0;
+    if (!tr && copy_l) {
+        BH_HIP(hipMemcpy2DAsync(Y, (size_t)n * sizeof(T), B, (size_t)ldb * sizeof(T), (size_t)n * sizeof(T),
+                                (size_t)nrhs, hipMemcpyDeviceToDevice, st));
+    } else if (!tr) {
         carve(pl);
         stt = part_sweep<T, kLN>(pl, a, true, st);
         if (!stt) stt = part_sweep<T, kLN>(pl, a, false, st);
@@ -543,7 +551,7 @@ int part_solve(char trans, i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 lda
     a.growth += 1;
     const PartPlan &p2 = tr ? pl : pu;
     carve(p2);
-    stt = tr ? part_sweep<T, kLT>(p2, a, true, st) : part_sweep<T, kUN>(p2, a, true, st);
+    if (!(tr && copy_l)) stt = tr ? part_sweep<T, kLT>(p2, a, true, st) : part_sweep<T, kUN>(p2, a, true, st);
     if (stt) return stt;
     unsigned gh[2];
     BH_HIP(hipMemcpyAsync(gh, a.growth - 1, sizeof gh, hipMemcpyDeviceToHost, st));
@@ -555,6 +563,11 @@ int part_solve(char trans, i64 n, i64 kl, i64 ku, i64 nrhs, const T *ab, i64 lda
             return bh_dgbtrs(tr ? 'T' : 'N', n, kl, ku, nrhs, (const double *)ab, ldab, ipiv, (double *)B, ldb);
         else
             return bh_sgbtrs(tr ? 'T' : 'N', n, kl, ku, nrhs, (const float *)ab, ldab, ipiv, (float *)B, ldb);
+    }
+    if (tr && copy_l) {
+        BH_HIP(hipMemcpy2DAsync(B, (size_t)ldb * sizeof(T), Y, (size_t)n * sizeof(T), (size_t)n * sizeof(T),
+                                (size_t)nrhs, hipMemcpyDeviceToDevice, st));
+        return BH_OK;
     }
     return tr ? part_sweep<T, kLT>(p2, a, false, st) : part_sweep<T, kUN>(p2, a, false, st);
 }
